@@ -1,0 +1,171 @@
+/*
+ * rgf.h -- C ABI of the MI355X (gfx950) recursive Green's function solver.
+ *
+ * Drop-in compute layer for qttools' GF solver plugin:
+ *   GFSolver.selected_inv / selected_solve  (src/qttools/greens_function_solver/solver.py:31-113)
+ *   RGF                                      (src/qttools/greens_function_solver/rgf.py:10-438)
+ *   linalg.inv                               (src/qttools/kernels/linalg/inv.py:24-40)
+ * The Python class quatrex_amd.greens_function_solver.RGF binds these symbols
+ * with ctypes and keeps the reference's Python signatures.
+ *
+ * Conventions
+ *  - Complex values are interleaved (re, im): complex128 = 2 x double,
+ *    complex64 = 2 x float -- the numpy/torch memory layout.
+ *  - All matrix pointers are DEVICE pointers; host arrays are marked "host".
+ *  - Block-tridiagonal operands use the packed layout: for block sizes
+ *    b_0..b_{N-1}, block (i,i) (b_i x b_i, row-major) sits at element offset
+ *    sum_{k<i} b_k^2 of `diag`; blocks (i,i+1) (b_i x b_{i+1}) and (i+1,i)
+ *    (b_{i+1} x b_i) sit at offset sum_{k<i} b_k b_{k+1} of `upper` and
+ *    `lower`.  Energies are the slowest axis (stride = total diag / offdiag
+ *    size).  These are the dense blocks of DSDBSparse.blocks[i, j]
+ *    (src/qttools/datastructures/dsdbsparse.py:1285-1363).
+ *  - Calls are asynchronous on the context's HIP stream; results are ready
+ *    after the stream is synchronised.  A singular pivot does not stop the
+ *    sweep (outputs then hold inf/NaN, as with cupy); rgf_status() reports it.
+ *  - No C++ exception crosses this interface; every call returns an rgf_status.
+ */
+#ifndef QUATREX_AMD_RGF_H
+#define QUATREX_AMD_RGF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the Python layer maps them to the reference's exceptions:
+ * EINVAL -> ValueError/TypeError, ESINGULAR -> numpy.linalg.LinAlgError). */
+#define RGF_OK 0
+#define RGF_EINVAL 1
+#define RGF_ESINGULAR 2
+#define RGF_EHIP 3
+#define RGF_ENOMEM 4
+#define RGF_EUNSUPPORTED 5
+
+/* Options for rgf_set_option. */
+#define RGF_OPT_MAX_CHUNK 1        /* max energies per internal sweep (0 = auto)           */
+#define RGF_OPT_WORKSPACE_BUDGET 2 /* bytes the context may allocate for workspace (auto)  */
+#define RGF_OPT_TILE_CFG 3         /* GEMM tile config for all block sizes (-1 = autotune) */
+
+typedef struct rgf_ctx rgf_ctx;
+
+/* Block partition of one device: n_blocks blocks of block_sizes[i] (host array). */
+typedef struct rgf_geometry {
+  int32_t n_blocks;
+  const int64_t* block_sizes;
+} rgf_geometry;
+
+/* Packed block-tridiagonal operand (device pointers, see Conventions). */
+typedef struct rgf_bt_const {
+  const void* diag;
+  const void* upper;
+  const void* lower;
+} rgf_bt_const;
+
+typedef struct rgf_bt {
+  void* diag;
+  void* upper;
+  void* lower;
+} rgf_bt;
+
+/* Open-boundary blocks, the content of OBCBlocks
+ * (src/qttools/greens_function_solver/solver.py:12-28): each member is NULL
+ * or a host array of n_blocks device pointers, NULL entries meaning "no OBC
+ * on this block"; a non-NULL entry i points to an (n_energies, b_i, b_i)
+ * complex128 array.  RGF applies A_ii - B^R_i and Sigma^<>_ii + B^<>_i
+ * (rgf.py:216-262). */
+typedef struct rgf_obc {
+  const void* const* retarded;
+  const void* const* lesser;
+  const void* const* greater;
+} rgf_obc;
+
+/* Mixed-precision options for rgf_selected_solve_mixed. */
+typedef struct rgf_mixed_opts {
+  int32_t refine;            /* FP64 residual refinement steps per inverted block (>= 0) */
+  double fallback_tol;       /* FP64 re-solve when an energy's refinement residual exceeds this
+                                (<= 0 disables the fallback) */
+  int32_t* fallback_flags;   /* optional device int32[n_energies]: 1 where the FP64 fallback ran */
+  double* residuals;         /* optional device double[n_energies]: max relative refinement residual */
+} rgf_mixed_opts;
+
+/* Context: one per device and stream.  stream = NULL uses the null stream.
+ * Replaces the implicit cupy device/stream state of the reference. */
+int rgf_create(int device, void* stream, rgf_ctx** out);
+int rgf_destroy(rgf_ctx* ctx);
+int rgf_set_stream(rgf_ctx* ctx, void* stream);
+int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value);
+
+/* Copies the last error message of ctx (NUL-terminated) into buf. */
+int rgf_last_error(const rgf_ctx* ctx, char* buf, size_t len);
+
+/* Synchronises the stream; info (host int64[3]) receives (energy, block,
+ * column) of the first exactly-zero pivot since the last call, or -1s.
+ * Returns RGF_ESINGULAR in that case, like numpy.linalg.inv raising
+ * LinAlgError (src/qttools/kernels/linalg/inv.py:6-11). */
+int rgf_status(rgf_ctx* ctx, int64_t* info);
+
+/* Workspace (bytes) the context needs to run a sweep over n_energies
+ * energies at once (selected_solve when solve != 0, with current and
+ * retarded-output choices as given); the admission check of the serving
+ * loop.  No allocation happens. */
+int rgf_workspace_size(const rgf_geometry* geom, int64_t n_energies, int32_t solve,
+                       int32_t with_current, int32_t with_retarded, size_t* bytes);
+
+/* Caller-owned workspace (e.g. a torch tensor); NULL/0 returns to
+ * context-owned workspace.  The context never frees caller memory. */
+int rgf_set_workspace(rgf_ctx* ctx, void* ptr, size_t bytes);
+
+/* Batched inverse of n_mats contiguous n x n complex128 matrices: out = a^{-1}.
+ * Replaces linalg.inv (src/qttools/kernels/linalg/inv.py:24-40).  a is not
+ * modified.  n <= 1024. */
+int rgf_inv_z(rgf_ctx* ctx, int64_t n, int64_t n_mats, const void* a, void* out);
+
+/* Selected inversion X = A^{-1} on the tridiagonal pattern, complex128.
+ * Replaces RGF.selected_inv (rgf.py:25-113).  obc_retarded: NULL or a host
+ * array of n_blocks device pointers (see rgf_obc).  x->upper / x->lower may be
+ * NULL to skip those outputs. */
+int rgf_selected_inv_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                       const rgf_bt_const* a, const void* const* obc_retarded, rgf_bt* x);
+
+/* Selected solve, complex128 throughout.  Replaces RGF.selected_solve
+ * (rgf.py:115-438):
+ *   G^R = (A - B^R)^{-1},  G^<> = G^R (Sigma^<> + B^<>) G^R^H
+ * on the tridiagonal pattern.  Only sigma->diag and sigma->upper are read
+ * (skew-Hermitian inputs, solver.py:87-91).
+ *  g_retarded: NULL, or diag (and optionally upper/lower) outputs.  The
+ *              reference writes diagonal G^R only; upper/lower extend it.
+ *  g_lesser / g_greater: diag and upper required; lower optional (written as
+ *              -G^<_{i,i+1}^H, rgf.py:357-359; pass NULL for symmetric storage).
+ *              Stored diagonal blocks are (X - X^H)/2 (rgf.py:305-310).
+ *  current:    NULL, or device complex128[n_energies][n_blocks+1]: the
+ *              Meir-Wingreen layer currents (rgf.py:388-428); then obc->lesser
+ *              and obc->greater must hold blocks 0 and n_blocks-1.
+ */
+int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                         const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
+                         const rgf_bt_const* sigma_greater, const rgf_obc* obc,
+                         rgf_bt* g_retarded, rgf_bt* g_lesser, rgf_bt* g_greater, void* current);
+
+/* Mixed-precision selected solve: same interface and outputs (complex128) as
+ * rgf_selected_solve_z; the Schur-complement GEMMs and G Sigma G^H triple
+ * products run in complex64 on FP32 matrix cores with FP32 accumulation,
+ * the diagonal-block inversions in FP64, followed by `opts->refine` FP64
+ * residual-refinement steps per inverted block; energies whose refinement
+ * residual exceeds opts->fallback_tol are re-solved in FP64.  No reference
+ * counterpart: the precision mode of this build's serving path. */
+int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                             const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
+                             const rgf_bt_const* sigma_greater, const rgf_obc* obc,
+                             const rgf_mixed_opts* opts, rgf_bt* g_retarded, rgf_bt* g_lesser,
+                             rgf_bt* g_greater, void* current);
+
+/* Library version string. */
+const char* rgf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUATREX_AMD_RGF_H */

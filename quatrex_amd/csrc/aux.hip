@@ -1,0 +1,114 @@
+// Elementwise and reduction kernels around the RGF sweeps.
+#include "qx_common.h"
+
+namespace qx {
+
+namespace {
+
+// X <- (X - X^H)/2 in place, for nblk diagonal blocks (offset offs[i], size
+// sizes[i]) of each energy.  Each thread owns one (r, c) pair with r <= c.
+__global__ void __launch_bounds__(256)
+project_kernel(double2* base, int64_t sE, const int64_t* offs, const int* sizes) {
+  const int blk = blockIdx.y, e = blockIdx.z;
+  const int b = sizes[blk];
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= b * b) return;
+  const int r = idx / b, c = idx % b;
+  if (r > c) return;
+  double2* X = base + (int64_t)e * sE + offs[blk];
+  const double2 u = X[(int64_t)r * b + c];
+  const double2 l = X[(int64_t)c * b + r];
+  // (u - conj(l))/2 at (r,c); (l - conj(u))/2 at (c,r)
+  const double2 nu = cmk(0.5 * (u.x - l.x), 0.5 * (u.y + l.y));
+  X[(int64_t)r * b + c] = nu;
+  if (r != c) X[(int64_t)c * b + r] = cmk(-nu.x, nu.y);
+}
+
+// out[e*sOut + slot] = sign * (tr(X1 Y1) - tr(X2 Y2)),  tr(XY) = sum_rc X[r][c] Y[c][r].
+__global__ void __launch_bounds__(256)
+trace_kernel(const TraceJob* jobs, double2* out, int64_t sOut) {
+  const TraceJob J = jobs[blockIdx.x];
+  const int e = blockIdx.y;
+  const int b = J.b;
+  const double2* X1 = J.X1 + (int64_t)e * J.sX1;
+  const double2* Y1 = J.Y1 + (int64_t)e * J.sY1;
+  const double2* X2 = J.X2 + (int64_t)e * J.sX2;
+  const double2* Y2 = J.Y2 + (int64_t)e * J.sY2;
+  double sr = 0.0, si = 0.0;
+  for (int idx = threadIdx.x; idx < b * b; idx += blockDim.x) {
+    const int r = idx / b, c = idx % b;
+    const double2 p = cmul(X1[idx], Y1[(int64_t)c * b + r]);
+    const double2 q = cmul(X2[idx], Y2[(int64_t)c * b + r]);
+    sr += p.x - q.x;
+    si += p.y - q.y;
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = sr;
+  red[1][threadIdx.x] = si;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(int64_t)e * sOut + J.slot] = cmk(J.sign * red[0][0], J.sign * red[1][0]);
+}
+
+__global__ void z2c_kernel(const double2* src, int64_t sSrc, float2* dst, int64_t sDst,
+                           int64_t count) {
+  const int e = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = src[(int64_t)e * sSrc + i];
+    dst[(int64_t)e * sDst + i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+__global__ void c2z_kernel(const float2* src, int64_t sSrc, double2* dst, int64_t sDst,
+                           int64_t count) {
+  const int e = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 v = src[(int64_t)e * sSrc + i];
+    dst[(int64_t)e * sDst + i] = cmk((double)v.x, (double)v.y);
+  }
+}
+
+}  // namespace
+
+hipError_t launch_project(double2* base, int64_t sE, const int64_t* offs_dev, const int* sizes_dev,
+                          int nblk, int bmax, int nE, hipStream_t stream) {
+  if (nblk == 0 || nE == 0) return hipSuccess;
+  dim3 grid((bmax * bmax + 255) / 256, nblk, nE);
+  hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, stream, base, sE, offs_dev, sizes_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_traces(const TraceJob* jobs_dev, int njobs, int nE, double2* out, int64_t sOut,
+                         hipStream_t stream) {
+  if (njobs == 0 || nE == 0) return hipSuccess;
+  hipLaunchKernelGGL(trace_kernel, dim3(njobs, nE), dim3(256), 0, stream, jobs_dev, out, sOut);
+  return hipGetLastError();
+}
+
+hipError_t launch_z2c(const double2* src, int64_t sSrc, float2* dst, int64_t sDst, int64_t count,
+                      int nE, hipStream_t stream) {
+  if (count == 0 || nE == 0) return hipSuccess;
+  int gx = (int)((count + 255) / 256);
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(z2c_kernel, dim3(gx, nE), dim3(256), 0, stream, src, sSrc, dst, sDst, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_c2z(const float2* src, int64_t sSrc, double2* dst, int64_t sDst, int64_t count,
+                      int nE, hipStream_t stream) {
+  if (count == 0 || nE == 0) return hipSuccess;
+  int gx = (int)((count + 255) / 256);
+  if (gx > 1024) gx = 1024;
+  hipLaunchKernelGGL(c2z_kernel, dim3(gx, nE), dim3(256), 0, stream, src, sSrc, dst, sDst, count);
+  return hipGetLastError();
+}
+
+}  // namespace qx

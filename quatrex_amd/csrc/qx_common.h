@@ -1,0 +1,167 @@
+// Shared device helpers and launch descriptors for the RGF kernels (gfx950).
+//
+// Complex numbers are interleaved (re, im) doubles, i.e. the memory layout of
+// numpy/torch complex128, handled as HIP double2.  Matrices are row-major with
+// a leading dimension and a per-energy stride (in complex elements).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qx {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__host__ __device__ inline double2 cmk(double re, double im) { return make_double2(re, im); }
+__host__ __device__ inline double2 cadd(double2 a, double2 b) { return cmk(a.x + b.x, a.y + b.y); }
+__host__ __device__ inline double2 csub(double2 a, double2 b) { return cmk(a.x - b.x, a.y - b.y); }
+__host__ __device__ inline double2 cmul(double2 a, double2 b) {
+  return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ inline double2 cconj(double2 a) { return cmk(a.x, -a.y); }
+__host__ __device__ inline bool cnz(double2 a) { return a.x != 0.0 || a.y != 0.0; }
+// 1/d via conj(d)/|d|^2 with a power-of-two-free scaling by max(|re|,|im|).
+__host__ __device__ inline double2 crecip(double2 d) {
+  double s = fmax(fabs(d.x), fabs(d.y));
+  double xr = d.x / s, xi = d.y / s;
+  double den = s * (xr * xr + xi * xi);
+  return cmk(xr / den, -xi / den);
+}
+
+enum Op : int32_t { kOpN = 0, kOpC = 1 };
+enum D2Mode : int32_t { kD2None = 0, kD2Partial = 1, kD2Copy = 2, kD2ConjT = 3 };
+
+constexpr int kMaxProblems = 6;
+constexpr int kMaxAddends = 3;
+
+// One product  alpha * op(A) * op(B)  of a problem (op = N or conjugate transpose).
+struct ZTerm {
+  const double2* A;
+  const double2* B;
+  int64_t sA, sB;  // per-energy strides
+  int32_t lda, ldb, opA, opB, k, pad_;
+  double2 alpha;
+};
+
+// Elementwise addend  c1 * F[r][c] + c2 * conj(F[c][r]).
+struct ZAddend {
+  const double2* F;
+  int64_t sF;
+  int32_t ldf, pad_;
+  double2 c1, c2;
+};
+
+// D = sum_t term_t + sum_a addend_a   (m x n), with an optional second output:
+//   kD2Partial: D2 = value after the first `npre` addends,
+//   kD2Copy:    D2 = s2 * D,
+//   kD2ConjT:   D2^T = s2 * conj(D)  (i.e. D2[c][r] = s2 * conj(D[r][c])).
+struct ZProblem {
+  int32_t m, n, nterms, naddends, npre, d2mode;
+  ZTerm t[2];
+  ZAddend ad[kMaxAddends];
+  double2* D;
+  int64_t sD;
+  int32_t ldd, ldd2;
+  double2* D2;
+  int64_t sD2;
+  double2 s2;
+};
+
+// Up to kMaxProblems independent problems, each over nE energies, in one launch.
+struct ZGroup {
+  int32_t nprob, nE;
+  int32_t wg_base[kMaxProblems + 1];  // cumulative workgroup counts
+  int32_t tiles_n[kMaxProblems];
+  int32_t tiles_mn[kMaxProblems];
+  ZProblem p[kMaxProblems];
+};
+
+// Single-precision twin of ZProblem for the mixed-precision sweeps: operands
+// are complex64 (float2), accumulation is FP32 on MFMA.
+struct CTerm {
+  const float2* A;
+  const float2* B;
+  int64_t sA, sB;
+  int32_t lda, ldb, opA, opB, k, pad_;
+  float2 alpha;
+};
+struct CAddend {
+  const float2* F;
+  int64_t sF;
+  int32_t ldf, pad_;
+  float2 c1, c2;
+};
+struct CProblem {
+  int32_t m, n, nterms, naddends, npre, d2mode;
+  CTerm t[2];
+  CAddend ad[kMaxAddends];
+  float2* D;
+  int64_t sD;
+  int32_t ldd, ldd2;
+  float2* D2;
+  int64_t sD2;
+  float2 s2;
+};
+struct CGroup {
+  int32_t nprob, nE;
+  int32_t wg_base[kMaxProblems + 1];
+  int32_t tiles_n[kMaxProblems];
+  int32_t tiles_mn[kMaxProblems];
+  CProblem p[kMaxProblems];
+};
+
+// Tile configurations selectable per launch (keyed by block size by the autotuner).
+enum TileCfg : int32_t {
+  kTile64x64 = 0,  // 256 threads, 4 waves of 32x32
+  kTile32x32 = 1,  // 64 threads, 1 wave of 32x32
+  kTile64x32 = 2,  // 128 threads, 2 waves of 32x32
+  kTile128x64 = 3, // 512 threads, 8 waves of 32x32
+  kNumTileCfgs = 4
+};
+int tile_m(int cfg);
+int tile_n(int cfg);
+
+// XCD-aware remap: hardware dispatches consecutive workgroups round-robin over
+// the 8 XCDs; give each XCD a contiguous range of logical ids so tiles of one
+// energy's matrices share that XCD's L2.
+__device__ inline int xcd_remap(int w, int total) {
+  const int q = total >> 3, r = total & 7;
+  const int x = w & 7, s = w >> 3;
+  return x * q + (x < r ? x : r) + s;
+}
+
+// Host-side launchers (defined in the .hip translation units).
+hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream);
+hipError_t launch_cgemm(const CGroup& g, int cfg, hipStream_t stream);
+
+// Batched Gauss-Jordan inversion with partial pivoting, in place in S
+// (n x n, ld n, energy stride sS); result (column-permuted back) written to
+// Out (ld ldo, stride sOut).  aux must hold zinv_aux_bytes(n) per energy.
+size_t zinv_aux_bytes(int n);
+hipError_t launch_zinv(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t sOut,
+                       int ldo, void* aux, int* info, int info_tag, int64_t e0, int tile_cfg,
+                       hipStream_t stream);
+
+// In-place anti-Hermitian projection X <- (X - X^H)/2 of nblk diagonal blocks.
+hipError_t launch_project(double2* base, int64_t sE, const int64_t* offs_dev, const int* sizes_dev,
+                          int nblk, int bmax, int nE, hipStream_t stream);
+
+// Meir-Wingreen current traces.
+struct TraceJob {
+  const double2* X1; const double2* Y1;  // + tr(X1 Y1)
+  const double2* X2; const double2* Y2;  // - tr(X2 Y2)
+  int64_t sX1, sY1, sX2, sY2;
+  int32_t b, slot;
+  double sign;
+};
+hipError_t launch_traces(const TraceJob* jobs_dev, int njobs, int nE, double2* out, int64_t sOut,
+                         hipStream_t stream);
+
+// Precision conversion helpers for the mixed path.
+hipError_t launch_z2c(const double2* src, int64_t sSrc, float2* dst, int64_t sDst, int64_t count,
+                      int nE, hipStream_t stream);
+hipError_t launch_c2z(const float2* src, int64_t sSrc, double2* dst, int64_t sDst, int64_t count,
+                      int nE, hipStream_t stream);
+
+}  // namespace qx

@@ -1,0 +1,893 @@
+// C ABI and host-side sweep driver for the RGF selected inversion / solve.
+//
+// The driver turns one call into a sequence of energy-batched launches on the
+// context stream: grouped complex GEMMs (zgemm.hip), blocked Gauss-Jordan
+// inversions (zinv.hip) and the projection / current kernels (aux.hip).  All
+// energies of a chunk advance through the same block step together, so every
+// launch covers (problems x energies x tiles) workgroups.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rgf.h"
+#include "qx_common.h"
+#include "rgf_driver.h"
+
+using namespace qx;
+
+namespace qx {
+
+const char* status_name(int s) {
+  switch (s) {
+    case RGF_OK: return "ok";
+    case RGF_EINVAL: return "invalid argument";
+    case RGF_ESINGULAR: return "singular matrix";
+    case RGF_EHIP: return "HIP error";
+    case RGF_ENOMEM: return "out of device memory";
+    default: return "unsupported";
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+Ctx::~Ctx() {
+  if (ws_owned && ws) (void)hipFree(ws);
+  if (meta) (void)hipFree(meta);
+  if (info_dev) (void)hipFree(info_dev);
+}
+
+int Ctx::fail(int code, const std::string& msg) {
+  last_error = msg;
+  return code;
+}
+
+int Ctx::hip(hipError_t e, const char* what) {
+  if (e == hipSuccess) return RGF_OK;
+  last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return e == hipErrorOutOfMemory ? RGF_ENOMEM : RGF_EHIP;
+}
+
+int Ctx::ensure_workspace(size_t bytes, char** out) {
+  if (bytes <= ws_bytes && ws) {
+    *out = static_cast<char*>(ws);
+    return RGF_OK;
+  }
+  if (!ws_owned && ws) {
+    return fail(RGF_ENOMEM, "caller workspace too small: need " + std::to_string(bytes) +
+                                " bytes, have " + std::to_string(ws_bytes));
+  }
+  if (ws) {
+    // the old buffer may still be in use by queued work
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip(e, "hipStreamSynchronize");
+    (void)hipFree(ws);
+    ws = nullptr;
+    ws_bytes = 0;
+  }
+  hipError_t e = hipMalloc(&ws, bytes);
+  if (e != hipSuccess) {
+    ws = nullptr;
+    return hip(e, "hipMalloc(workspace)");
+  }
+  ws_bytes = bytes;
+  ws_owned = true;
+  *out = static_cast<char*>(ws);
+  return RGF_OK;
+}
+
+int Ctx::ensure_meta(size_t bytes) {
+  if (bytes <= meta_bytes && meta) return RGF_OK;
+  if (meta) {
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip(e, "hipStreamSynchronize");
+    (void)hipFree(meta);
+    meta = nullptr;
+  }
+  size_t want = std::max(bytes, (size_t)1 << 16);
+  hipError_t e = hipMalloc(&meta, want);
+  if (e != hipSuccess) {
+    meta = nullptr;
+    return hip(e, "hipMalloc(meta)");
+  }
+  meta_bytes = want;
+  return RGF_OK;
+}
+
+int Ctx::tile_for(int b) const {
+  if (tile_override >= 0) return tile_override;
+  if (b >= 0 && b < (int)tile_table.size() && tile_table[b] >= 0) return tile_table[b];
+  return b < 64 ? kTile32x32 : kTile64x64;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry
+// ---------------------------------------------------------------------------
+
+int Geo::init(const rgf_geometry* g, std::string* err) {
+  if (!g || g->n_blocks < 1 || !g->block_sizes) {
+    *err = "geometry: n_blocks >= 1 and block_sizes required";
+    return RGF_EINVAL;
+  }
+  N = g->n_blocks;
+  bs.assign(g->block_sizes, g->block_sizes + N);
+  od.assign(N + 1, 0);
+  ou.assign(N, 0);
+  bmax = 0;
+  for (int i = 0; i < N; ++i) {
+    if (bs[i] < 1 || bs[i] > 1024) {
+      *err = "geometry: block sizes must be in [1, 1024]";
+      return RGF_EUNSUPPORTED;
+    }
+    od[i + 1] = od[i] + bs[i] * bs[i];
+    if (i + 1 < N) ou[i + 1] = ou[i] + bs[i] * bs[i + 1];
+    bmax = std::max<int64_t>(bmax, bs[i]);
+  }
+  Dsz = od[N];
+  Usz = N > 1 ? ou[N - 1] : 0;
+  return RGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped GEMM helper
+// ---------------------------------------------------------------------------
+
+PB::PB(int m, int n) {
+  std::memset(&P, 0, sizeof(P));
+  P.m = m;
+  P.n = n;
+  P.s2 = cmk(1.0, 0.0);
+  P.npre = -1;
+}
+PB& PB::term(Mat A, int opA, Mat B, int opB, int k, double2 alpha) {
+  ZTerm& t = P.t[P.nterms++];
+  t.A = A.p; t.sA = A.s; t.lda = A.ld; t.opA = opA;
+  t.B = B.p; t.sB = B.s; t.ldb = B.ld; t.opB = opB;
+  t.k = k; t.alpha = alpha;
+  return *this;
+}
+PB& PB::add(Mat F, double2 c1, double2 c2) {
+  if (!F.p) return *this;
+  ZAddend& a = P.ad[P.naddends++];
+  a.F = F.p; a.sF = F.s; a.ldf = F.ld; a.c1 = c1; a.c2 = c2;
+  return *this;
+}
+PB& PB::out(Mat D) {
+  P.D = D.p; P.sD = D.s; P.ldd = D.ld;
+  return *this;
+}
+PB& PB::out2(int mode, Mat D2, double2 s2) {
+  if (!D2.p) return *this;
+  P.d2mode = mode; P.D2 = D2.p; P.sD2 = D2.s; P.ldd2 = D2.ld; P.s2 = s2;
+  return *this;
+}
+PB& PB::partial_here() {
+  P.npre = P.naddends;
+  return *this;
+}
+ZProblem PB::done() const {
+  ZProblem q = P;
+  if (q.npre < 0) q.npre = q.naddends;
+  if (q.d2mode != kD2Partial && q.npre > q.naddends) q.npre = q.naddends;
+  return q;
+}
+
+int Ctx::gemm(const std::vector<ZProblem>& probs, int nE, int b) {
+  size_t i = 0;
+  while (i < probs.size()) {
+    ZGroup g;
+    std::memset(&g, 0, sizeof(g));
+    g.nE = nE;
+    int np = 0;
+    while (i < probs.size() && np < kMaxProblems) {
+      if (probs[i].D) g.p[np++] = probs[i];
+      ++i;
+    }
+    g.nprob = np;
+    if (np == 0) continue;
+    int rc = hip(launch_zgemm(g, tile_for(b), stream), "zgemm launch");
+    if (rc) return rc;
+  }
+  return RGF_OK;
+}
+
+int Ctx::inv(double2* S, int64_t sS, int n, int nE, Mat out, char* aux, int tag, int64_t e0) {
+  return hip(launch_zinv(S, sS, n, nE, out.p, out.s, out.ld, aux, info_dev, tag, e0, tile_for(n),
+                         stream),
+             "zinv launch");
+}
+
+}  // namespace qx
+
+// ---------------------------------------------------------------------------
+// Sweeps (FP64)
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr double2 ONE = {1.0, 0.0};
+constexpr double2 MONE = {-1.0, 0.0};
+constexpr double2 ZERO = {0.0, 0.0};
+
+struct Work {
+  double2* slots = nullptr;  // [E][kSlots][bmax^2]
+  int64_t slot_elems = 0;
+  int64_t sE = 0;
+  double2* gr = nullptr;     // [E][Dsz] retarded diagonal storage when not an output
+  double2* stl = nullptr;    // [E][Dsz] Sigma~^< per block (current)
+  double2* stg = nullptr;
+  char* aux = nullptr;       // [E][aux_bytes]
+  Mat slot(int q) const { return Mat{slots + q * slot_elems, sE, 0}; }
+};
+
+constexpr int kSlots = 11;
+
+size_t work_bytes(const Geo& g, int64_t E, bool solve, bool current, bool have_gr) {
+  const size_t bb = (size_t)g.bmax * g.bmax * sizeof(double2);
+  size_t per = (solve ? kSlots : 4) * bb + zinv_aux_bytes((int)g.bmax);
+  if (!have_gr) per += g.Dsz * sizeof(double2);
+  if (current) per += 2 * g.Dsz * sizeof(double2);
+  return per * (size_t)E + 256;
+}
+
+void carve(const Geo& g, int64_t E, bool solve, bool current, bool have_gr, char* base, Work* w) {
+  const int64_t bb = g.bmax * g.bmax;
+  const int nsl = solve ? kSlots : 4;
+  char* p = base;
+  w->slots = reinterpret_cast<double2*>(p);
+  w->slot_elems = bb;
+  w->sE = nsl * bb;
+  p += (size_t)E * nsl * bb * sizeof(double2);
+  if (!have_gr) {
+    w->gr = reinterpret_cast<double2*>(p);
+    p += (size_t)E * g.Dsz * sizeof(double2);
+  }
+  if (current) {
+    w->stl = reinterpret_cast<double2*>(p);
+    p += (size_t)E * g.Dsz * sizeof(double2);
+    w->stg = reinterpret_cast<double2*>(p);
+    p += (size_t)E * g.Dsz * sizeof(double2);
+  }
+  w->aux = p;
+}
+
+// Energy-offset views of packed operands.
+struct BTV {
+  const Geo* g;
+  double2* d;
+  double2* u;
+  double2* l;
+  int64_t e0;
+  Mat D(int i) const {
+    return d ? Mat{d + e0 * g->Dsz + g->od[i], g->Dsz, (int)g->bs[i]} : Mat{};
+  }
+  Mat U(int i) const {
+    return u ? Mat{u + e0 * g->Usz + g->ou[i], g->Usz, (int)g->bs[i + 1]} : Mat{};
+  }
+  Mat L(int i) const {
+    return l ? Mat{l + e0 * g->Usz + g->ou[i], g->Usz, (int)g->bs[i]} : Mat{};
+  }
+};
+
+BTV view(const Geo& g, const void* d, const void* u, const void* l, int64_t e0) {
+  return BTV{&g, (double2*)d, (double2*)u, (double2*)l, e0};
+}
+
+Mat obc_mat(const void* const* arr, const Geo& g, int i, int64_t e0) {
+  if (!arr || !arr[i]) return Mat{};
+  const int64_t b = g.bs[i];
+  return Mat{(double2*)arr[i] + e0 * b * b, b * b, (int)b};
+}
+
+Mat with_ld(Mat m, int ld) {
+  m.ld = ld;
+  return m;
+}
+
+// Forward elimination shared by selected_inv and selected_solve: for block j
+// writes g_j = (A~_jj - A_ji g_i A_ij)^{-1} into GR(j).
+int sweep_chunk_inv(Ctx* c, const Geo& g, int64_t e0, int E, const BTV& A,
+                    const void* const* obcr, const BTV& X, const Work& w) {
+  const int N = g.N;
+  auto S = [&](int b) { return with_ld(w.slot(0), b); };
+  auto P = [&](int b) { return with_ld(w.slot(1), b); };
+  int rc;
+  // block 0
+  {
+    const int b0 = (int)g.bs[0];
+    rc = c->gemm({PB(b0, b0).add(A.D(0), ONE).add(obc_mat(obcr, g, 0, e0), MONE).out(S(b0)).done()},
+                 E, b0);
+    if (rc) return rc;
+    rc = c->inv(S(b0).p, S(b0).s, b0, E, X.D(0), w.aux, 1, e0);
+    if (rc) return rc;
+  }
+  for (int j = 1; j < N; ++j) {
+    const int i = j - 1;
+    const int bi = (int)g.bs[i], bj = (int)g.bs[j];
+    rc = c->gemm({PB(bj, bi).term(A.L(i), kOpN, X.D(i), kOpN, bi).out(P(bi)).done()}, E, bj);
+    if (rc) return rc;
+    rc = c->gemm({PB(bj, bj)
+                      .term(P(bi), kOpN, A.U(i), kOpN, bi, MONE)
+                      .add(A.D(j), ONE)
+                      .add(obc_mat(obcr, g, j, e0), MONE)
+                      .out(S(bj))
+                      .done()},
+                 E, bj);
+    if (rc) return rc;
+    rc = c->inv(S(bj).p, S(bj).s, bj, E, X.D(j), w.aux, j + 1, e0);
+    if (rc) return rc;
+  }
+  // backward (rgf.py:98-110)
+  for (int i = N - 2; i >= 0; --i) {
+    const int j = i + 1;
+    const int bi = (int)g.bs[i], bj = (int)g.bs[j];
+    Mat U = with_ld(w.slot(2), bj);  // g_i A_ij        (bi x bj)
+    Mat R = with_ld(w.slot(3), bi);  // X_jj A_ji       (bj x bi)
+    Mat Xu = X.U(i), Xl = X.L(i);
+    Mat Xl_tmp = with_ld(w.slot(1), bi);
+    if (!Xl.p) Xl = Xl_tmp;
+    rc = c->gemm({PB(bi, bj).term(X.D(i), kOpN, A.U(i), kOpN, bi).out(U).done(),
+                  PB(bj, bi).term(X.D(j), kOpN, A.L(i), kOpN, bj).out(R).done()},
+                 E, bi);
+    if (rc) return rc;
+    std::vector<ZProblem> l2;
+    l2.push_back(PB(bj, bi).term(R, kOpN, X.D(i), kOpN, bi, MONE).out(Xl).done());
+    if (Xu.p) l2.push_back(PB(bi, bj).term(U, kOpN, X.D(j), kOpN, bj, MONE).out(Xu).done());
+    rc = c->gemm(l2, E, bi);
+    if (rc) return rc;
+    rc = c->gemm({PB(bi, bi).term(U, kOpN, Xl, kOpN, bj, MONE).add(X.D(i), ONE).out(X.D(i)).done()},
+                 E, bi);
+    if (rc) return rc;
+  }
+  return RGF_OK;
+}
+
+struct SolveIO {
+  BTV A, SL, SG;
+  const rgf_obc* obc;
+  BTV GR, GL, GG;
+  double2* current;  // [E_total][N+1] or null
+};
+
+int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io, const Work& w) {
+  const int N = g.N;
+  const BTV& A = io.A;
+  const BTV& SL = io.SL;
+  const BTV& SG = io.SG;
+  const BTV& GL = io.GL;
+  const BTV& GG = io.GG;
+  BTV GR = io.GR;
+  if (!GR.d) GR = BTV{&g, w.gr, nullptr, nullptr, 0};
+  const void* const* obr = io.obc ? io.obc->retarded : nullptr;
+  const void* const* obl = io.obc ? io.obc->lesser : nullptr;
+  const void* const* obg = io.obc ? io.obc->greater : nullptr;
+  auto OR = [&](int i) { return obc_mat(obr, g, i, e0); };
+  auto OL = [&](int i) { return obc_mat(obl, g, i, e0); };
+  auto OG = [&](int i) { return obc_mat(obg, g, i, e0); };
+  auto ST = [&](double2* base, int i) {
+    return base ? Mat{base + g.od[i], g.Dsz, (int)g.bs[i]} : Mat{};
+  };
+  auto sl = [&](int q, int ld) { return with_ld(w.slot(q), ld); };
+  int rc;
+
+  // ---- block 0 (rgf.py:216-239) ----
+  {
+    const int b = (int)g.bs[0];
+    Mat S = sl(0, b), Vl = sl(1, b), Vg = sl(2, b);
+    rc = c->gemm({PB(b, b).add(A.D(0), ONE).add(OR(0), MONE).out(S).done()}, E, b);
+    if (rc) return rc;
+    rc = c->inv(S.p, S.s, b, E, GR.D(0), w.aux, 1, e0);
+    if (rc) return rc;
+    PB vl(b, b), vg(b, b);
+    vl.term(GR.D(0), kOpN, SL.D(0), kOpN, b);
+    vg.term(GR.D(0), kOpN, SG.D(0), kOpN, b);
+    if (OL(0).p) vl.term(GR.D(0), kOpN, OL(0), kOpN, b);
+    if (OG(0).p) vg.term(GR.D(0), kOpN, OG(0), kOpN, b);
+    rc = c->gemm({vl.out(Vl).done(), vg.out(Vg).done()}, E, b);
+    if (rc) return rc;
+    rc = c->gemm({PB(b, b).term(Vl, kOpN, GR.D(0), kOpC, b).out(GL.D(0)).done(),
+                  PB(b, b).term(Vg, kOpN, GR.D(0), kOpC, b).out(GG.D(0)).done()},
+                 E, b);
+    if (rc) return rc;
+  }
+
+  // ---- forward sweep (rgf.py:241-302) ----
+  for (int j = 1; j < N; ++j) {
+    const int i = j - 1;
+    const int bi = (int)g.bs[i], bj = (int)g.bs[j];
+    Mat S = sl(0, bj), P = sl(1, bi), Ql = sl(2, bj), Qg = sl(3, bj), Yl = sl(4, bi),
+        Yg = sl(5, bi), Wl = sl(6, bj), Wg = sl(7, bj);
+    // P = A_ji g_i ; Y = A_ji g^<>_i
+    rc = c->gemm({PB(bj, bi).term(A.L(i), kOpN, GR.D(i), kOpN, bi).out(P).done(),
+                  PB(bj, bi).term(A.L(i), kOpN, GL.D(i), kOpN, bi).out(Yl).done(),
+                  PB(bj, bi).term(A.L(i), kOpN, GG.D(i), kOpN, bi).out(Yg).done()},
+                 E, bj);
+    if (rc) return rc;
+    // S = A_jj - B^R_j - P A_ij ; Q = P Sigma_ij
+    rc = c->gemm({PB(bj, bj)
+                      .term(P, kOpN, A.U(i), kOpN, bi, MONE)
+                      .add(A.D(j), ONE)
+                      .add(OR(j), MONE)
+                      .out(S)
+                      .done(),
+                  PB(bj, bj).term(P, kOpN, SL.U(i), kOpN, bi).out(Ql).done(),
+                  PB(bj, bj).term(P, kOpN, SG.U(i), kOpN, bi).out(Qg).done()},
+                 E, bj);
+    if (rc) return rc;
+    // W = Y A_ji^H + Q^H - Q  [-> Sigma~_j]  + Sigma_jj + B_j
+    rc = c->gemm({PB(bj, bj)
+                      .term(Yl, kOpN, A.L(i), kOpC, bi)
+                      .add(Ql, MONE, ONE)
+                      .partial_here()
+                      .add(SL.D(j), ONE)
+                      .add(OL(j), ONE)
+                      .out(Wl)
+                      .out2(kD2Partial, ST(w.stl, j))
+                      .done(),
+                  PB(bj, bj)
+                      .term(Yg, kOpN, A.L(i), kOpC, bi)
+                      .add(Qg, MONE, ONE)
+                      .partial_here()
+                      .add(SG.D(j), ONE)
+                      .add(OG(j), ONE)
+                      .out(Wg)
+                      .out2(kD2Partial, ST(w.stg, j))
+                      .done()},
+                 E, bj);
+    if (rc) return rc;
+    rc = c->inv(S.p, S.s, bj, E, GR.D(j), w.aux, j + 1, e0);
+    if (rc) return rc;
+    // V = g_j W ; g^<>_j = V g_j^H   (V reuses the Y slots)
+    Mat Vl = sl(4, bj), Vg = sl(5, bj);
+    rc = c->gemm({PB(bj, bj).term(GR.D(j), kOpN, Wl, kOpN, bj).out(Vl).done(),
+                  PB(bj, bj).term(GR.D(j), kOpN, Wg, kOpN, bj).out(Vg).done()},
+                 E, bj);
+    if (rc) return rc;
+    rc = c->gemm({PB(bj, bj).term(Vl, kOpN, GR.D(j), kOpC, bj).out(GL.D(j)).done(),
+                  PB(bj, bj).term(Vg, kOpN, GR.D(j), kOpC, bj).out(GG.D(j)).done()},
+                 E, bj);
+    if (rc) return rc;
+  }
+
+  // ---- backward sweep (rgf.py:315-412) ----
+  for (int i = N - 2; i >= 0; --i) {
+    const int j = i + 1;
+    const int bi = (int)g.bs[i], bj = (int)g.bs[j];
+    Mat U = sl(0, bj), R = sl(1, bi), M = sl(2, bj), K = sl(3, bi), Zl = sl(4, bj),
+        Zg = sl(5, bj), T2l = sl(6, bj), T2g = sl(7, bj), T1l = sl(8, bi), T1g = sl(9, bi),
+        Gn = sl(10, bi);
+    // U = g_i A_ij ; R = G_jj A_ji
+    rc = c->gemm({PB(bi, bj).term(GR.D(i), kOpN, A.U(i), kOpN, bi).out(U).done(),
+                  PB(bj, bi).term(GR.D(j), kOpN, A.L(i), kOpN, bj).out(R).done()},
+                 E, bi);
+    if (rc) return rc;
+    // M = U G_jj (G^R_ij = -M) ; K = U R ; Z = g_i Sigma_ij ; T2 = U G^<>_jj
+    rc = c->gemm({PB(bi, bj).term(U, kOpN, GR.D(j), kOpN, bj).out(M).out2(kD2Copy, GR.U(i), MONE).done(),
+                  PB(bi, bi).term(U, kOpN, R, kOpN, bj).out(K).done(),
+                  PB(bi, bj).term(GR.D(i), kOpN, SL.U(i), kOpN, bi).out(Zl).done(),
+                  PB(bi, bj).term(GR.D(i), kOpN, SG.U(i), kOpN, bi).out(Zg).done(),
+                  PB(bi, bj).term(U, kOpN, GL.D(j), kOpN, bj).out(T2l).done(),
+                  PB(bi, bj).term(U, kOpN, GG.D(j), kOpN, bj).out(T2g).done()},
+                 E, bi);
+    if (rc) return rc;
+    // T1 = K g^<>_i - Z M^H ; G^<>_ij = -g^<>_i R^H + Z G_jj^H - T2 ; G_new = g_i + K g_i ;
+    // G^R_ji = -R g_i
+    std::vector<ZProblem> l3;
+    l3.push_back(PB(bi, bi)
+                     .term(K, kOpN, GL.D(i), kOpN, bi)
+                     .term(Zl, kOpN, M, kOpC, bj, MONE)
+                     .out(T1l)
+                     .done());
+    l3.push_back(PB(bi, bi)
+                     .term(K, kOpN, GG.D(i), kOpN, bi)
+                     .term(Zg, kOpN, M, kOpC, bj, MONE)
+                     .out(T1g)
+                     .done());
+    l3.push_back(PB(bi, bj)
+                     .term(GL.D(i), kOpN, R, kOpC, bi, MONE)
+                     .term(Zl, kOpN, GR.D(j), kOpC, bj)
+                     .add(T2l, MONE)
+                     .out(GL.U(i))
+                     .out2(kD2ConjT, GL.L(i), MONE)
+                     .done());
+    l3.push_back(PB(bi, bj)
+                     .term(GG.D(i), kOpN, R, kOpC, bi, MONE)
+                     .term(Zg, kOpN, GR.D(j), kOpC, bj)
+                     .add(T2g, MONE)
+                     .out(GG.U(i))
+                     .out2(kD2ConjT, GG.L(i), MONE)
+                     .done());
+    l3.push_back(PB(bi, bi).term(K, kOpN, GR.D(i), kOpN, bi).add(GR.D(i), ONE).out(Gn).done());
+    if (GR.L(i).p)
+      l3.push_back(PB(bj, bi).term(R, kOpN, GR.D(i), kOpN, bi, MONE).out(GR.L(i)).done());
+    rc = c->gemm(l3, E, bi);
+    if (rc) return rc;
+    // G^<>_ii = g^<>_i + T2 U^H + (T1 - T1^H) ; G_ii = G_new
+    rc = c->gemm({PB(bi, bi)
+                      .term(T2l, kOpN, U, kOpC, bj)
+                      .add(GL.D(i), ONE)
+                      .add(T1l, ONE, MONE)
+                      .out(GL.D(i))
+                      .done(),
+                  PB(bi, bi)
+                      .term(T2g, kOpN, U, kOpC, bj)
+                      .add(GG.D(i), ONE)
+                      .add(T1g, ONE, MONE)
+                      .out(GG.D(i))
+                      .done(),
+                  PB(bi, bi).add(Gn, ONE).out(GR.D(i)).done()},
+                 E, bi);
+    if (rc) return rc;
+  }
+
+  // ---- current (rgf.py:388-428), from carried unprojected G^<> ----
+  if (io.current) {
+    std::vector<TraceJob> jobs;
+    auto job = [&](Mat X1, Mat Y1, Mat X2, Mat Y2, int b, int slot, double sign) {
+      TraceJob t{};
+      t.X1 = X1.p; t.Y1 = Y1.p; t.X2 = X2.p; t.Y2 = Y2.p;
+      t.sX1 = X1.s; t.sY1 = Y1.s; t.sX2 = X2.s; t.sY2 = Y2.s;
+      t.b = b; t.slot = slot; t.sign = sign;
+      jobs.push_back(t);
+    };
+    for (int j = 1; j < N; ++j)
+      job(ST(w.stg, j), GL.D(j), GG.D(j), ST(w.stl, j), (int)g.bs[j], j, 1.0);
+    job(OG(0), GL.D(0), GG.D(0), OL(0), (int)g.bs[0], 0, 1.0);
+    job(OG(N - 1), GL.D(N - 1), GG.D(N - 1), OL(N - 1), (int)g.bs[N - 1], N, -1.0);
+    const size_t bytes = jobs.size() * sizeof(TraceJob);
+    if (c->meta_used + bytes > c->meta_bytes) return c->fail(RGF_EHIP, "trace job area too small");
+    TraceJob* dev = reinterpret_cast<TraceJob*>(static_cast<char*>(c->meta) + c->meta_used);
+    rc = c->hip(hipMemcpyAsync(dev, jobs.data(), bytes, hipMemcpyHostToDevice, c->stream),
+                "copy trace jobs");
+    if (rc) return rc;
+    rc = c->hip(hipStreamSynchronize(c->stream), "sync trace jobs");  // host vector lifetime
+    if (rc) return rc;
+    rc = c->hip(launch_traces(dev, (int)jobs.size(), E, io.current + e0 * (N + 1), N + 1,
+                              c->stream),
+                "trace launch");
+    if (rc) return rc;
+  }
+
+  // ---- stored diagonal G^<> = (X - X^H)/2 (rgf.py:305-310, 362-364, 384-386) ----
+  rc = c->hip(launch_project(GL.d + e0 * g.Dsz, g.Dsz, c->offs_dev, c->sizes_dev, N, (int)g.bmax, E,
+                             c->stream),
+              "project launch");
+  if (rc) return rc;
+  rc = c->hip(launch_project(GG.d + e0 * g.Dsz, g.Dsz, c->offs_dev, c->sizes_dev, N, (int)g.bmax, E,
+                             c->stream),
+              "project launch");
+  return rc;
+}
+
+}  // namespace
+
+namespace qx {
+
+__global__ void zero_ints_kernel(int* p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+int Ctx::reset_info() {
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, stream, info_dev, 4);
+  return hip(hipGetLastError(), "zero info");
+}
+
+// Uploads block offsets/sizes used by the projection kernel (meta area, start).
+int Ctx::upload_geometry(const Geo& g) {
+  const size_t bytes = g.N * (sizeof(int64_t) + sizeof(int));
+  // geometry tables first, then room for one chunk's trace jobs
+  int rc = ensure_meta(bytes + 256 + (g.N + 2) * sizeof(TraceJob) + 4096);
+  if (rc) return rc;
+  std::vector<char> host(bytes);
+  std::memcpy(host.data(), g.od.data(), g.N * sizeof(int64_t));
+  std::vector<int> sz(g.N);
+  for (int i = 0; i < g.N; ++i) sz[i] = (int)g.bs[i];
+  std::memcpy(host.data() + g.N * sizeof(int64_t), sz.data(), g.N * sizeof(int));
+  rc = hip(hipMemcpyAsync(meta, host.data(), bytes, hipMemcpyHostToDevice, stream), "geometry upload");
+  if (rc) return rc;
+  rc = hip(hipStreamSynchronize(stream), "geometry upload sync");
+  if (rc) return rc;
+  offs_dev = reinterpret_cast<int64_t*>(meta);
+  sizes_dev = reinterpret_cast<int*>(static_cast<char*>(meta) + g.N * sizeof(int64_t));
+  meta_used = (bytes + 255) & ~(size_t)255;
+  return RGF_OK;
+}
+
+int64_t Ctx::chunk_for(const Geo& g, int64_t nE, bool solve, bool current, bool have_gr) {
+  int64_t chunk = nE;
+  if (max_chunk > 0) chunk = std::min<int64_t>(chunk, max_chunk);
+  const size_t per = work_bytes(g, 1, solve, current, have_gr);
+  size_t budget = ws_budget;
+  if (!ws_owned && ws) budget = ws_bytes;
+  int64_t fit = (int64_t)(budget / std::max<size_t>(per, 1));
+  if (fit < 1) fit = 1;
+  chunk = std::min(chunk, fit);
+  // keep gridDim.y/z and workgroup counts in range
+  chunk = std::min<int64_t>(chunk, 65535);
+  return std::max<int64_t>(chunk, 1);
+}
+
+}  // namespace qx
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+struct rgf_ctx : public qx::Ctx {};
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                              \
+  }                                                            \
+  catch (const std::bad_alloc&) {                              \
+    return ctx ? ctx->fail(RGF_ENOMEM, "host allocation failed") : RGF_ENOMEM; \
+  }                                                            \
+  catch (...) {                                                \
+    return ctx ? ctx->fail(RGF_EHIP, "internal error") : RGF_EHIP; \
+  }
+
+extern "C" {
+
+const char* rgf_version(void) { return "quatrex_amd-rgf 0.1 (gfx950)"; }
+
+int rgf_create(int device, void* stream, rgf_ctx** out) {
+  if (!out) return RGF_EINVAL;
+  *out = nullptr;
+  rgf_ctx* ctx = nullptr;
+  try {
+    ctx = new rgf_ctx();
+  } catch (...) {
+    return RGF_ENOMEM;
+  }
+  ctx->device = device;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  if (hipSetDevice(device) != hipSuccess) {
+    delete ctx;
+    return RGF_EHIP;
+  }
+  if (hipMalloc(&ctx->info_dev, 4 * sizeof(int)) != hipSuccess) {
+    delete ctx;
+    return RGF_ENOMEM;
+  }
+  if (hipMemset(ctx->info_dev, 0, 4 * sizeof(int)) != hipSuccess) {
+    delete ctx;
+    return RGF_EHIP;
+  }
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+    ctx->ws_budget = std::min<size_t>((size_t)16 << 30, free_b / 4);
+  *out = ctx;
+  return RGF_OK;
+}
+
+int rgf_destroy(rgf_ctx* ctx) {
+  if (!ctx) return RGF_OK;
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  else (void)hipDeviceSynchronize();
+  delete ctx;
+  return RGF_OK;
+}
+
+int rgf_set_stream(rgf_ctx* ctx, void* stream) {
+  if (!ctx) return RGF_EINVAL;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return RGF_OK;
+}
+
+int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value) {
+  if (!ctx) return RGF_EINVAL;
+  switch (option) {
+    case RGF_OPT_MAX_CHUNK: ctx->max_chunk = value; return RGF_OK;
+    case RGF_OPT_WORKSPACE_BUDGET:
+      if (value <= 0) return ctx->fail(RGF_EINVAL, "workspace budget must be positive");
+      ctx->ws_budget = (size_t)value;
+      return RGF_OK;
+    case RGF_OPT_TILE_CFG:
+      if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
+      ctx->tile_override = (int)value;
+      return RGF_OK;
+    default:
+      // 1000 + b: tile config for block size b (autotuner table)
+      if (option >= 1000 && option < 1000 + 1025) {
+        const int b = option - 1000;
+        if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
+        if ((int)ctx->tile_table.size() <= b) ctx->tile_table.resize(b + 1, -1);
+        ctx->tile_table[b] = (int)value;
+        return RGF_OK;
+      }
+      return ctx->fail(RGF_EINVAL, "unknown option");
+  }
+}
+
+int rgf_last_error(const rgf_ctx* ctx, char* buf, size_t len) {
+  if (!ctx || !buf || len == 0) return RGF_EINVAL;
+  std::snprintf(buf, len, "%s", ctx->last_error.c_str());
+  return RGF_OK;
+}
+
+int rgf_status(rgf_ctx* ctx, int64_t* info) {
+  if (!ctx) return RGF_EINVAL;
+  int h[4] = {0, 0, 0, 0};
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return ctx->hip(e, "hipStreamSynchronize");
+  e = hipMemcpy(h, ctx->info_dev, sizeof(h), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return ctx->hip(e, "read status");
+  e = hipMemset(ctx->info_dev, 0, sizeof(h));
+  if (e != hipSuccess) return ctx->hip(e, "reset status");
+  if (info) {
+    info[0] = h[0] ? h[1] : -1;
+    info[1] = h[0] ? h[0] - 1 : -1;
+    info[2] = h[0] ? h[2] : -1;
+  }
+  if (h[0]) {
+    ctx->last_error = "Singular matrix (energy " + std::to_string(h[1]) + ", block " +
+                      std::to_string(h[0] - 1) + ", column " + std::to_string(h[2]) + ")";
+    return RGF_ESINGULAR;
+  }
+  return RGF_OK;
+}
+
+int rgf_workspace_size(const rgf_geometry* geom, int64_t n_energies, int32_t solve,
+                       int32_t with_current, int32_t with_retarded, size_t* bytes) {
+  if (!bytes || n_energies < 0) return RGF_EINVAL;
+  qx::Geo g;
+  std::string err;
+  int rc = g.init(geom, &err);
+  if (rc) return rc;
+  *bytes = work_bytes(g, n_energies, solve != 0, with_current != 0, with_retarded != 0);
+  return RGF_OK;
+}
+
+int rgf_set_workspace(rgf_ctx* ctx, void* ptr, size_t bytes) {
+  if (!ctx) return RGF_EINVAL;
+  if (ctx->ws_owned && ctx->ws) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->ws);
+  }
+  ctx->ws = ptr;
+  ctx->ws_bytes = ptr ? bytes : 0;
+  ctx->ws_owned = false;
+  return RGF_OK;
+}
+
+int rgf_inv_z(rgf_ctx* ctx, int64_t n, int64_t n_mats, const void* a, void* out) {
+  if (!ctx) return RGF_EINVAL;
+  GUARD_BEGIN
+  if (n < 1 || n > 1024) return ctx->fail(RGF_EUNSUPPORTED, "rgf_inv_z: n must be in [1, 1024]");
+  if (n_mats < 0 || (n_mats > 0 && (!a || !out))) return ctx->fail(RGF_EINVAL, "rgf_inv_z: null buffer");
+  if (n_mats == 0) return RGF_OK;
+  int rc = hipSetDevice(ctx->device) == hipSuccess ? RGF_OK : RGF_EHIP;
+  if (rc) return ctx->fail(rc, "hipSetDevice");
+  const size_t per = (size_t)n * n * sizeof(double2) + qx::zinv_aux_bytes((int)n);
+  int64_t chunk = std::max<int64_t>(1, (int64_t)(ctx->ws_budget / per));
+  if (!ctx->ws_owned && ctx->ws) chunk = std::max<int64_t>(1, (int64_t)(ctx->ws_bytes / per));
+  chunk = std::min<int64_t>({chunk, n_mats, 65535});
+  char* ws = nullptr;
+  rc = ctx->ensure_workspace(per * chunk + 256, &ws);
+  if (rc) return rc;
+  rc = ctx->reset_info();
+  if (rc) return rc;
+  const double2* A = static_cast<const double2*>(a);
+  double2* O = static_cast<double2*>(out);
+  double2* S = reinterpret_cast<double2*>(ws);
+  char* aux = ws + (size_t)chunk * n * n * sizeof(double2);
+  for (int64_t e0 = 0; e0 < n_mats; e0 += chunk) {
+    const int E = (int)std::min<int64_t>(chunk, n_mats - e0);
+    rc = ctx->hip(hipMemcpyAsync(S, A + e0 * n * n, (size_t)E * n * n * sizeof(double2),
+                                 hipMemcpyDeviceToDevice, ctx->stream),
+                  "copy to workspace");
+    if (rc) return rc;
+    rc = ctx->inv(S, n * n, (int)n, E, qx::Mat{O + e0 * n * n, n * n, (int)n}, aux, 1, e0);
+    if (rc) return rc;
+  }
+  return RGF_OK;
+  GUARD_END
+}
+
+int rgf_selected_inv_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                       const rgf_bt_const* a, const void* const* obc_retarded, rgf_bt* x) {
+  if (!ctx) return RGF_EINVAL;
+  GUARD_BEGIN
+  qx::Geo g;
+  int rc = g.init(geom, &ctx->last_error);
+  if (rc) return rc;
+  if (n_energies < 0) return ctx->fail(RGF_EINVAL, "n_energies < 0");
+  if (n_energies == 0) return RGF_OK;
+  if (!a || !a->diag || (g.N > 1 && (!a->upper || !a->lower)))
+    return ctx->fail(RGF_EINVAL, "selected_inv: A needs diag, upper and lower");
+  if (!x || !x->diag) return ctx->fail(RGF_EINVAL, "selected_inv: X diag output required");
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  const int64_t chunk = ctx->chunk_for(g, n_energies, false, false, true);
+  char* ws = nullptr;
+  rc = ctx->ensure_workspace(work_bytes(g, chunk, false, false, true), &ws);
+  if (rc) return rc;
+  rc = ctx->reset_info();
+  if (rc) return rc;
+  Work w;
+  carve(g, chunk, false, false, true, ws, &w);
+  for (int64_t e0 = 0; e0 < n_energies; e0 += chunk) {
+    const int E = (int)std::min<int64_t>(chunk, n_energies - e0);
+    BTV A = view(g, a->diag, a->upper, a->lower, e0);
+    BTV X = view(g, x->diag, x->upper, x->lower, e0);
+    rc = sweep_chunk_inv(ctx, g, e0, E, A, obc_retarded, X, w);
+    if (rc) return rc;
+  }
+  return RGF_OK;
+  GUARD_END
+}
+
+int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                         const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
+                         const rgf_bt_const* sigma_greater, const rgf_obc* obc,
+                         rgf_bt* g_retarded, rgf_bt* g_lesser, rgf_bt* g_greater, void* current) {
+  if (!ctx) return RGF_EINVAL;
+  GUARD_BEGIN
+  qx::Geo g;
+  int rc = g.init(geom, &ctx->last_error);
+  if (rc) return rc;
+  if (n_energies < 0) return ctx->fail(RGF_EINVAL, "n_energies < 0");
+  if (n_energies == 0) return RGF_OK;
+  const bool multi = g.N > 1;
+  if (!a || !a->diag || (multi && (!a->upper || !a->lower)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: A needs diag, upper and lower");
+  if (!sigma_lesser || !sigma_greater || !sigma_lesser->diag || !sigma_greater->diag ||
+      (multi && (!sigma_lesser->upper || !sigma_greater->upper)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: Sigma needs diag and upper");
+  if (!g_lesser || !g_greater || !g_lesser->diag || !g_greater->diag ||
+      (multi && (!g_lesser->upper || !g_greater->upper)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: G^< and G^> need diag and upper outputs");
+  if (current) {
+    const int last = g.N - 1;
+    if (!obc || !obc->lesser || !obc->greater || !obc->lesser[0] || !obc->greater[0] ||
+        !obc->lesser[last] || !obc->greater[last])
+      return ctx->fail(RGF_EINVAL,
+                       "current needs lesser/greater OBC blocks on the first and last block");
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  const bool have_gr = g_retarded && g_retarded->diag;
+  const int64_t chunk = ctx->chunk_for(g, n_energies, true, current != nullptr, have_gr);
+  rc = ctx->upload_geometry(g);
+  if (rc) return rc;
+  char* ws = nullptr;
+  rc = ctx->ensure_workspace(work_bytes(g, chunk, true, current != nullptr, have_gr), &ws);
+  if (rc) return rc;
+  rc = ctx->reset_info();
+  if (rc) return rc;
+  Work w;
+  carve(g, chunk, true, current != nullptr, have_gr, ws, &w);
+  for (int64_t e0 = 0; e0 < n_energies; e0 += chunk) {
+    const int E = (int)std::min<int64_t>(chunk, n_energies - e0);
+    SolveIO io;
+    io.A = view(g, a->diag, a->upper, a->lower, e0);
+    io.SL = view(g, sigma_lesser->diag, sigma_lesser->upper, nullptr, e0);
+    io.SG = view(g, sigma_greater->diag, sigma_greater->upper, nullptr, e0);
+    io.obc = obc;
+    io.GR = have_gr ? view(g, g_retarded->diag, g_retarded->upper, g_retarded->lower, e0)
+                    : BTV{&g, nullptr, nullptr, nullptr, 0};
+    io.GL = view(g, g_lesser->diag, g_lesser->upper, g_lesser->lower, e0);
+    io.GG = view(g, g_greater->diag, g_greater->upper, g_greater->lower, e0);
+    io.current = static_cast<double2*>(current);
+    // workspace-resident retarded storage is indexed from the chunk start
+    rc = sweep_chunk_solve(ctx, g, e0, E, io, w);
+    if (rc) return rc;
+  }
+  return RGF_OK;
+  GUARD_END
+}
+
+int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                             const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
+                             const rgf_bt_const* sigma_greater, const rgf_obc* obc,
+                             const rgf_mixed_opts* opts, rgf_bt* g_retarded, rgf_bt* g_lesser,
+                             rgf_bt* g_greater, void* current) {
+  if (!ctx) return RGF_EINVAL;
+  (void)geom; (void)n_energies; (void)a; (void)sigma_lesser; (void)sigma_greater; (void)obc;
+  (void)opts; (void)g_retarded; (void)g_lesser; (void)g_greater; (void)current;
+  return ctx->fail(RGF_EUNSUPPORTED, "mixed precision path not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// Host-side driver types shared by the FP64 and mixed-precision sweeps.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/rgf.h"
+#include "qx_common.h"
+
+namespace qx {
+
+// Energy-strided matrix view: element (r, c) of energy e is p[e*s + r*ld + c].
+struct Mat {
+  double2* p = nullptr;
+  int64_t s = 0;
+  int ld = 0;
+};
+
+struct Geo {
+  int N = 0;
+  std::vector<int64_t> bs, od, ou;
+  int64_t Dsz = 0, Usz = 0, bmax = 0;
+  int init(const rgf_geometry* g, std::string* err);
+};
+
+// Builder for one grouped-GEMM problem.
+struct PB {
+  ZProblem P;
+  PB(int m, int n);
+  PB& term(Mat A, int opA, Mat B, int opB, int k, double2 alpha = double2{1.0, 0.0});
+  PB& add(Mat F, double2 c1, double2 c2 = double2{0.0, 0.0});
+  PB& out(Mat D);
+  PB& out2(int mode, Mat D2, double2 s2 = double2{1.0, 0.0});
+  PB& partial_here();
+  ZProblem done() const;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  bool ws_owned = true;
+  size_t ws_budget = (size_t)4 << 30;
+  int64_t max_chunk = 0;
+  int tile_override = -1;
+  std::vector<int> tile_table;
+  int* info_dev = nullptr;
+  void* meta = nullptr;
+  size_t meta_bytes = 0;
+  size_t meta_used = 0;
+  int64_t* offs_dev = nullptr;
+  int* sizes_dev = nullptr;
+  std::string last_error;
+
+  virtual ~Ctx();
+  int fail(int code, const std::string& msg);
+  int hip(hipError_t e, const char* what);
+  int ensure_workspace(size_t bytes, char** out);
+  int ensure_meta(size_t bytes);
+  int tile_for(int b) const;
+  int gemm(const std::vector<ZProblem>& probs, int nE, int b);
+  int inv(double2* S, int64_t sS, int n, int nE, Mat out, char* aux, int tag, int64_t e0);
+  int reset_info();
+  int upload_geometry(const Geo& g);
+  int64_t chunk_for(const Geo& g, int64_t nE, bool solve, bool current, bool have_gr);
+};
+
+}  // namespace qx

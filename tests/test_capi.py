@@ -1,0 +1,61 @@
+"""C-ABI checks that need no GPU: the library loads and exports exactly the
+entry points declared in include/rgf.h."""
+
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from quatrex_amd import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "rgf.h"
+
+
+def header_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rgf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_matches_binding_list():
+    assert header_functions() == sorted(_lib.EXPORTED_SYMBOLS)
+
+
+def test_library_built_and_exports_header_symbols():
+    assert _lib.LIB_PATH.exists(), "run python -m quatrex_amd.build"
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    for name in header_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\sT\s(rgf_[a-z0-9_]+)$", out, flags=re.M))
+    assert set(header_functions()) <= exported
+
+
+def test_binding_declares_signatures():
+    lib = _lib.load()
+    assert lib.rgf_version().decode().startswith("quatrex_amd-rgf")
+    assert lib.rgf_selected_solve_z.argtypes is not None
+
+
+def test_workspace_query_without_gpu():
+    from quatrex_amd.layout import Geometry
+    from quatrex_amd.solve import workspace_size
+
+    g = Geometry.uniform(32, 256)
+    one = workspace_size(g, 1)
+    many = workspace_size(g, 64)
+    assert one > 11 * 256 * 256 * 16
+    assert many > 60 * one
+    assert workspace_size(g, 1, current=True) > one
+    with pytest.raises(_lib.RgfError):
+        workspace_size(Geometry.uniform(2, 2000), 1)
+
+
+def test_null_context_is_rejected():
+    lib = _lib.load()
+    assert lib.rgf_selected_solve_z(None, None, 0, None, None, None, None, None, None, None,
+                                    None) == _lib.RGF_EINVAL
