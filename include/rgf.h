@@ -47,6 +47,8 @@ extern "C" {
 #define RGF_OPT_MAX_CHUNK 1        /* max energies per internal sweep (0 = auto)           */
 #define RGF_OPT_WORKSPACE_BUDGET 2 /* bytes the context may allocate for workspace (auto)  */
 #define RGF_OPT_TILE_CFG 3         /* GEMM tile config for all block sizes (-1 = autotune) */
+#define RGF_OPT_PROFILE 4          /* 1: time every sweep GEMM launch and every block inversion
+                                      with HIP events on the context stream (0: off)          */
 
 typedef struct rgf_ctx rgf_ctx;
 
@@ -160,6 +162,19 @@ int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_e
                              const rgf_bt_const* sigma_greater, const rgf_obc* obc,
                              const rgf_mixed_opts* opts, rgf_bt* g_retarded, rgf_bt* g_lesser,
                              rgf_bt* g_greater, void* current);
+
+/* Accumulated launch statistics while RGF_OPT_PROFILE is on (synchronises
+ * the stream).  Flops are algorithmic: 8*m*n*k real flops per complex
+ * product term, 8*n^3 per n x n inversion.  No reference counterpart. */
+typedef struct rgf_profile {
+  int64_t gemm_launches;
+  double gemm_ms;
+  double gemm_flops;
+  int64_t inv_calls;
+  double inv_ms;
+  double inv_flops;
+} rgf_profile;
+int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset);
 
 /* Library version string. */
 const char* rgf_version(void);

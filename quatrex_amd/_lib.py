@@ -23,6 +23,7 @@ RGF_EUNSUPPORTED = 5
 RGF_OPT_MAX_CHUNK = 1
 RGF_OPT_WORKSPACE_BUDGET = 2
 RGF_OPT_TILE_CFG = 3
+RGF_OPT_PROFILE = 4
 RGF_OPT_TILE_FOR_BLOCK = 1000  # + block size
 
 # Every function declared in include/rgf.h (checked by the tests).
@@ -39,6 +40,7 @@ EXPORTED_SYMBOLS = (
     "rgf_selected_inv_z",
     "rgf_selected_solve_z",
     "rgf_selected_solve_mixed",
+    "rgf_profile_read",
     "rgf_version",
 )
 
@@ -72,6 +74,17 @@ class MixedOpts_t(ctypes.Structure):
     ]
 
 
+class Profile_t(ctypes.Structure):
+    _fields_ = [
+        ("gemm_launches", c_int64),
+        ("gemm_ms", c_double),
+        ("gemm_flops", c_double),
+        ("inv_calls", c_int64),
+        ("inv_ms", c_double),
+        ("inv_flops", c_double),
+    ]
+
+
 class RgfError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"rgf error {code}: {msg}")
@@ -101,6 +114,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.rgf_selected_solve_mixed.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
                                              P(BtConst_t), P(BtConst_t), P(Obc_t),
                                              P(MixedOpts_t), P(Bt_t), P(Bt_t), P(Bt_t), c_void_p]
+    lib.rgf_profile_read.argtypes = [c_void_p, P(Profile_t), c_int32]
     lib.rgf_version.argtypes = []
     lib.rgf_version.restype = c_char_p
     for name in EXPORTED_SYMBOLS:

@@ -133,6 +133,7 @@ __device__ inline int xcd_remap(int w, int total) {
 
 // Host-side launchers (defined in the .hip translation units).
 hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream);
+hipError_t launch_zgemm_update(const ZGroup& g, int cfg, hipStream_t stream);
 hipError_t launch_cgemm(const CGroup& g, int cfg, hipStream_t stream);
 
 // Batched Gauss-Jordan inversion with partial pivoting, in place in S

@@ -36,6 +36,8 @@ const char* status_name(int s) {
 // ---------------------------------------------------------------------------
 
 Ctx::~Ctx() {
+  for (auto& t : timed) { event_pool.push_back(t.a); event_pool.push_back(t.b); }
+  for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
   if (ws_owned && ws) (void)hipFree(ws);
   if (meta) (void)hipFree(meta);
   if (info_dev) (void)hipFree(info_dev);
@@ -176,6 +178,60 @@ ZProblem PB::done() const {
   return q;
 }
 
+hipEvent_t Ctx::take_event() {
+  if (!event_pool.empty()) {
+    hipEvent_t e = event_pool.back();
+    event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int Ctx::prof_begin(hipEvent_t* a) {
+  *a = nullptr;
+  if (!profile) return RGF_OK;
+  *a = take_event();
+  if (!*a) return fail(RGF_EHIP, "hipEventCreate");
+  return hip(hipEventRecord(*a, stream), "hipEventRecord");
+}
+
+int Ctx::prof_end(hipEvent_t a, double flops, int kind) {
+  if (!profile || !a) return RGF_OK;
+  hipEvent_t b = take_event();
+  if (!b) return fail(RGF_EHIP, "hipEventCreate");
+  int rc = hip(hipEventRecord(b, stream), "hipEventRecord");
+  if (rc) return rc;
+  timed.push_back(Timed{a, b, flops, kind});
+  if (timed.size() >= 4096) return prof_collect();
+  return RGF_OK;
+}
+
+int Ctx::prof_collect() {
+  for (auto& t : timed) {
+    hipError_t e = hipEventSynchronize(t.b);
+    if (e != hipSuccess) return hip(e, "hipEventSynchronize");
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, t.a, t.b);
+    if (e != hipSuccess) return hip(e, "hipEventElapsedTime");
+    if (t.kind == 0) { prof_gemm_launches++; prof_gemm_ms += ms; prof_gemm_flops += t.flops; }
+    else { prof_inv_calls++; prof_inv_ms += ms; prof_inv_flops += t.flops; }
+    event_pool.push_back(t.a);
+    event_pool.push_back(t.b);
+  }
+  timed.clear();
+  return RGF_OK;
+}
+
+static double group_flops(const ZGroup& g) {
+  double f = 0;
+  for (int p = 0; p < g.nprob; ++p)
+    for (int t = 0; t < g.p[p].nterms; ++t)
+      f += 8.0 * g.p[p].m * (double)g.p[p].n * g.p[p].t[t].k;
+  return f * g.nE;
+}
+
 int Ctx::gemm(const std::vector<ZProblem>& probs, int nE, int b) {
   size_t i = 0;
   while (i < probs.size()) {
@@ -189,16 +245,26 @@ int Ctx::gemm(const std::vector<ZProblem>& probs, int nE, int b) {
     }
     g.nprob = np;
     if (np == 0) continue;
-    int rc = hip(launch_zgemm(g, tile_for(b), stream), "zgemm launch");
+    hipEvent_t ev;
+    int rc = prof_begin(&ev);
+    if (rc) return rc;
+    rc = hip(launch_zgemm(g, tile_for(b), stream), "zgemm launch");
+    if (rc) return rc;
+    rc = prof_end(ev, group_flops(g), 0);
     if (rc) return rc;
   }
   return RGF_OK;
 }
 
 int Ctx::inv(double2* S, int64_t sS, int n, int nE, Mat out, char* aux, int tag, int64_t e0) {
-  return hip(launch_zinv(S, sS, n, nE, out.p, out.s, out.ld, aux, info_dev, tag, e0, tile_for(n),
-                         stream),
-             "zinv launch");
+  hipEvent_t ev;
+  int rc = prof_begin(&ev);
+  if (rc) return rc;
+  rc = hip(launch_zinv(S, sS, n, nE, out.p, out.s, out.ld, aux, info_dev, tag, e0, tile_for(n),
+                       stream),
+           "zinv launch");
+  if (rc) return rc;
+  return prof_end(ev, 8.0 * n * (double)n * n * nE, 1);
 }
 
 }  // namespace qx
@@ -686,6 +752,7 @@ int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value) {
       if (value <= 0) return ctx->fail(RGF_EINVAL, "workspace budget must be positive");
       ctx->ws_budget = (size_t)value;
       return RGF_OK;
+    case RGF_OPT_PROFILE: ctx->profile = value != 0; return RGF_OK;
     case RGF_OPT_TILE_CFG:
       if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
       ctx->tile_override = (int)value;
@@ -877,6 +944,23 @@ int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energ
   }
   return RGF_OK;
   GUARD_END
+}
+
+int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset) {
+  if (!ctx || !out) return RGF_EINVAL;
+  int rc = ctx->prof_collect();
+  if (rc) return rc;
+  out->gemm_launches = ctx->prof_gemm_launches;
+  out->gemm_ms = ctx->prof_gemm_ms;
+  out->gemm_flops = ctx->prof_gemm_flops;
+  out->inv_calls = ctx->prof_inv_calls;
+  out->inv_ms = ctx->prof_inv_ms;
+  out->inv_flops = ctx->prof_inv_flops;
+  if (reset) {
+    ctx->prof_gemm_launches = ctx->prof_inv_calls = 0;
+    ctx->prof_gemm_ms = ctx->prof_gemm_flops = ctx->prof_inv_ms = ctx->prof_inv_flops = 0;
+  }
+  return RGF_OK;
 }
 
 int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,

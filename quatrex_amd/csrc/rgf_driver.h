@@ -53,6 +53,18 @@ struct Ctx {
   int* sizes_dev = nullptr;
   std::string last_error;
 
+  // launch profiling (RGF_OPT_PROFILE)
+  bool profile = false;
+  struct Timed { hipEvent_t a, b; double flops; int kind; };
+  std::vector<Timed> timed;
+  std::vector<hipEvent_t> event_pool;
+  int64_t prof_gemm_launches = 0, prof_inv_calls = 0;
+  double prof_gemm_ms = 0, prof_gemm_flops = 0, prof_inv_ms = 0, prof_inv_flops = 0;
+  hipEvent_t take_event();
+  int prof_begin(hipEvent_t* a);
+  int prof_end(hipEvent_t a, double flops, int kind);
+  int prof_collect();
+
   virtual ~Ctx();
   int fail(int code, const std::string& msg);
   int hip(hipError_t e, const char* what);

@@ -104,7 +104,9 @@ __device__ inline void store_b(const ZTerm& t, int tid, const double2 (&rb)[Tile
   }
 }
 
-template <int TM, int TN>
+// kRole only separates kernel names in traces: 0 = sweep products,
+// 1 = trailing updates inside the blocked inversion.
+template <int TM, int TN, int kRole>
 __global__ void __launch_bounds__((TileShape<TM, TN>::kThreads))
 zgemm_grouped_kernel(const ZGroup g) {
   using S = TileShape<TM, TN>;
@@ -220,7 +222,7 @@ zgemm_grouped_kernel(const ZGroup g) {
       }
 }
 
-template <int TM, int TN>
+template <int TM, int TN, int kRole>
 hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   ZGroup g = g0;
   int base = 0;
@@ -233,7 +235,7 @@ hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   }
   g.wg_base[g.nprob] = base;
   if (base == 0) return hipSuccess;
-  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN>), dim3(base), dim3(TileShape<TM, TN>::kThreads),
+  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole>), dim3(base), dim3(TileShape<TM, TN>::kThreads),
                      0, stream, g);
   return hipGetLastError();
 }
@@ -261,10 +263,17 @@ int tile_n(int cfg) {
 
 hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream) {
   switch (cfg) {
-    case kTile32x32: return launch_cfg<32, 32>(g, stream);
-    case kTile64x32: return launch_cfg<64, 32>(g, stream);
-    case kTile128x64: return launch_cfg<128, 64>(g, stream);
-    default: return launch_cfg<64, 64>(g, stream);
+    case kTile32x32: return launch_cfg<32, 32, 0>(g, stream);
+    case kTile64x32: return launch_cfg<64, 32, 0>(g, stream);
+    case kTile128x64: return launch_cfg<128, 64, 0>(g, stream);
+    default: return launch_cfg<64, 64, 0>(g, stream);
+  }
+}
+
+hipError_t launch_zgemm_update(const ZGroup& g, int cfg, hipStream_t stream) {
+  switch (cfg) {
+    case kTile32x32: return launch_cfg<32, 32, 1>(g, stream);
+    default: return launch_cfg<64, 64, 1>(g, stream);
   }
 }
 

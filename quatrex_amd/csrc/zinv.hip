@@ -282,7 +282,7 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
     add(0, k);
     add(k + nbe, n - k - nbe);
     g.nprob = np;
-    hipError_t err = launch_zgemm(g, tile_cfg, stream);
+    hipError_t err = launch_zgemm_update(g, tile_cfg, stream);
     if (err != hipSuccess) return err;
   }
   const int rows = (n >= 256) ? 4 : ((1024 + n - 1) / n);

@@ -88,6 +88,15 @@ class Context:
             _lib.check(self._lib.rgf_set_workspace(self._h, ctypes.c_void_p(tensor.data_ptr()),
                                                    nbytes), self._h)
 
+    def profile(self, on: bool = True) -> None:
+        """Time every sweep GEMM launch and block inversion with HIP events."""
+        self.set_option(_lib.RGF_OPT_PROFILE, 1 if on else 0)
+
+    def profile_read(self, reset: bool = True) -> dict:
+        p = _lib.Profile_t()
+        _lib.check(self._lib.rgf_profile_read(self._h, ctypes.byref(p), int(reset)), self._h)
+        return {k: getattr(p, k) for k, _ in _lib.Profile_t._fields_}
+
     def status(self) -> None:
         """Synchronise; raise ``LinAlgError`` if a pivot was exactly zero."""
         info = (ctypes.c_int64 * 3)()
