@@ -104,14 +104,37 @@ __device__ inline void store_b(const ZTerm& t, int tid, const double2 (&rb)[Tile
   }
 }
 
+template <int TM, int TN>
+struct Smem {
+  static constexpr int kA = TK * (TM + PAD);  // one LDS stage of op(A), k-major
+  static constexpr int kB = TK * (TN + PAD);  // one LDS stage of op(B), k-major
+  static constexpr int kMain = 2 * kA + 2 * kB;
+  static constexpr int kTile = TM * (TN + 1);
+  static constexpr int kTileT = TN * (TM + 1);
+  static constexpr int kSize = kMain > kTile ? (kMain > kTileT ? kMain : kTileT)
+                                             : (kTile > kTileT ? kTile : kTileT);
+};
+
+// f64 MFMA with the BLGP field used as the negate-A modifier.
+__device__ inline double4_t mfma(double a, double b, double4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline double4_t mfma_nega(double a, double b, double4_t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 1);
+}
+
 // kRole only separates kernel names in traces: 0 = sweep products,
 // 1 = trailing updates inside the blocked inversion.
 template <int TM, int TN, int kRole>
-__global__ void __launch_bounds__((TileShape<TM, TN>::kThreads))
+__global__ void __launch_bounds__((TileShape<TM, TN>::kThreads), (TM * TN > 4096 ? 1 : 2))
 zgemm_grouped_kernel(const ZGroup g) {
   using S = TileShape<TM, TN>;
-  __shared__ double2 As[2][TK][TM + PAD];
-  __shared__ double2 Bs[2][TK][TN + PAD];
+  using L = Smem<TM, TN>;
+  __shared__ double2 smem[L::kSize];
+  auto As = [&](int buf) { return reinterpret_cast<double2(*)[TM + PAD]>(smem + buf * L::kA); };
+  auto Bs = [&](int buf) {
+    return reinterpret_cast<double2(*)[TN + PAD]>(smem + 2 * L::kA + buf * L::kB);
+  };
 
   const int total = g.wg_base[g.nprob];
   const int wid = xcd_remap(blockIdx.x, total);
@@ -146,9 +169,8 @@ zgemm_grouped_kernel(const ZGroup g) {
     double2 ra[S::kAPer], rb[S::kBPer];
     load_a<TM, TN>(T, e, m, m0, 0, tid, ra);
     load_b<TM, TN>(T, e, n, n0, 0, tid, rb);
-    if (ti > 0) __syncthreads();  // previous term's last LDS reads are done
-    store_a<TM, TN>(T, tid, ra, As[0]);
-    store_b<TM, TN>(T, tid, rb, Bs[0]);
+    store_a<TM, TN>(T, tid, ra, As(0));
+    store_b<TM, TN>(T, tid, rb, Bs(0));
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
       const int cur = kt & 1;
@@ -157,69 +179,122 @@ zgemm_grouped_kernel(const ZGroup g) {
         load_a<TM, TN>(T, e, m, m0, (kt + 1) * TK, tid, ra);
         load_b<TM, TN>(T, e, n, n0, (kt + 1) * TK, tid, rb);
       }
+      auto as = As(cur);
+      auto bs = Bs(cur);
+      // fragments double-buffered across the k-steps of this LDS stage
+      double2 fa[2][2], fb[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) fa[0][i] = as[lane >> 4][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) fb[0][j] = bs[lane >> 4][wn * 32 + j * 16 + (lane & 15)];
 #pragma unroll
       for (int ks = 0; ks < TK / 4; ++ks) {
-        const int kk = ks * 4 + (lane >> 4);
-        double2 a[2], b[2];
+        const int c = ks & 1;
+        if (ks + 1 < TK / 4) {
+          const int kk = (ks + 1) * 4 + (lane >> 4);
 #pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = As[cur][kk][wm * 32 + i * 16 + (lane & 15)];
+          for (int i = 0; i < 2; ++i) fa[c ^ 1][i] = as[kk][wm * 32 + i * 16 + (lane & 15)];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = Bs[cur][kk][wn * 32 + j * 16 + (lane & 15)];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-            accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].x, accr[i][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j) fb[c ^ 1][j] = bs[kk][wn * 32 + j * 16 + (lane & 15)];
+        }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
-            acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].x, b[j].y, acci[i][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j) accr[i][j] = mfma(fa[c][i].x, fb[c][j].x, accr[i][j]);
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
-            accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[i].y, b[j].y, accr[i][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j) acci[i][j] = mfma(fa[c][i].x, fb[c][j].y, acci[i][j]);
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
-            acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i].y, b[j].x, acci[i][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j) accr[i][j] = mfma_nega(fa[c][i].y, fb[c][j].y, accr[i][j]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) acci[i][j] = mfma(fa[c][i].y, fb[c][j].x, acci[i][j]);
       }
       if (more) {
-        store_a<TM, TN>(T, tid, ra, As[cur ^ 1]);
-        store_b<TM, TN>(T, tid, rb, Bs[cur ^ 1]);
+        store_a<TM, TN>(T, tid, ra, As(cur ^ 1));
+        store_b<TM, TN>(T, tid, rb, Bs(cur ^ 1));
       }
       __syncthreads();
     }
   }
 
-  // Epilogue: for v_mfma_f64_16x16x4f64 register r of a lane holds
-  // D[4*r + lane/16][lane%16] of each 16x16 sub-tile (probed on gfx950).
-  double2* D = P.D + (int64_t)e * P.sD;
-  double2* D2 = P.D2 ? P.D2 + (int64_t)e * P.sD2 : nullptr;
+  // ---- epilogue, staged through LDS so global accesses are row-coalesced ----
+  // For v_mfma_f64_16x16x4f64 register r of a lane holds D[4*r + lane/16][lane%16]
+  // of each 16x16 sub-tile (probed on gfx950).
+  constexpr int kPer = TM * TN / S::kThreads;
+  auto Cs = reinterpret_cast<double2(*)[TN + 1]>(smem);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 32 + i * 16 + r * 4 + (lane >> 4);
-        const int col = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (row >= m || col >= n) continue;
-        double2 v = cmk(accr[i][j][r], acci[i][j][r]);
-        for (int a = 0; a < P.naddends; ++a) {
-          if (a == P.npre && P.d2mode == kD2Partial) D2[(int64_t)row * P.ldd2 + col] = v;
-          const ZAddend& A = P.ad[a];
-          const double2* F = A.F + (int64_t)e * A.sF;
-          if (cnz(A.c1)) v = cadd(v, cmul(A.c1, F[(int64_t)row * A.ldf + col]));
-          if (cnz(A.c2)) v = cadd(v, cmul(A.c2, cconj(F[(int64_t)col * A.ldf + row])));
-        }
-        if (P.npre >= P.naddends && P.d2mode == kD2Partial) D2[(int64_t)row * P.ldd2 + col] = v;
-        D[(int64_t)row * P.ldd + col] = v;
-        if (P.d2mode == kD2Copy) D2[(int64_t)row * P.ldd2 + col] = cmul(P.s2, v);
-        else if (P.d2mode == kD2ConjT) D2[(int64_t)col * P.ldd2 + row] = cmul(P.s2, cconj(v));
+      for (int r = 0; r < 4; ++r)
+        Cs[wm * 32 + i * 16 + r * 4 + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)] =
+            cmk(accr[i][j][r], acci[i][j][r]);
+  __syncthreads();
+  const int64_t eo = (int64_t)e;
+  double2* D = P.D + eo * P.sD;
+  double2* D2 = P.D2 ? P.D2 + eo * P.sD2 : nullptr;
+  // Each thread owns elements idx = tid + q*kThreads of the tile (row-major,
+  // so global accesses are coalesced along rows); values live in Cs and are
+  // updated in groups of four: loads of a group are issued before its stores.
+  for (int a = 0; a <= P.naddends; ++a) {
+    if (a == P.npre && P.d2mode == kD2Partial) {
+#pragma unroll 4
+      for (int q = 0; q < kPer; ++q) {
+        const int idx = tid + q * S::kThreads;
+        const int row = m0 + idx / TN, col = n0 + idx % TN;
+        if (row < m && col < n) D2[(int64_t)row * P.ldd2 + col] = Cs[idx / TN][idx % TN];
       }
+    }
+    if (a == P.naddends) break;
+    const ZAddend& A = P.ad[a];
+    const double2* F = A.F + eo * A.sF;
+    const bool use1 = cnz(A.c1), use2 = cnz(A.c2);
+#pragma unroll 1
+    for (int q0 = 0; q0 < kPer; q0 += 4) {
+      double2 f1[4], f2[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = tid + (q0 + u) * S::kThreads;
+        const int row = m0 + idx / TN, col = n0 + idx % TN;
+        const bool in = row < m && col < n;
+        f1[u] = (use1 && in) ? F[(int64_t)row * A.ldf + col] : cmk(0.0, 0.0);
+        f2[u] = (use2 && in) ? F[(int64_t)col * A.ldf + row] : cmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = tid + (q0 + u) * S::kThreads;
+        double2 v = Cs[idx / TN][idx % TN];
+        v = cadd(v, cadd(cmul(A.c1, f1[u]), cmul(A.c2, cconj(f2[u]))));
+        Cs[idx / TN][idx % TN] = v;
+      }
+    }
+  }
+#pragma unroll 4
+  for (int q = 0; q < kPer; ++q) {
+    const int idx = tid + q * S::kThreads;
+    const int row = m0 + idx / TN, col = n0 + idx % TN;
+    if (row < m && col < n) {
+      const double2 v = Cs[idx / TN][idx % TN];
+      D[(int64_t)row * P.ldd + col] = v;
+      if (P.d2mode == kD2Copy) D2[(int64_t)row * P.ldd2 + col] = cmul(P.s2, v);
+    }
+  }
+  if (P.d2mode == kD2ConjT) {
+    // D2[c][r] = s2 conj(D[r][c]): read the tile transposed from LDS, store rows of D2
+#pragma unroll 4
+    for (int q = 0; q < kPer; ++q) {
+      const int idx = tid + q * S::kThreads;
+      const int tr = idx / TM, tc = idx % TM;
+      const int row2 = n0 + tr, col2 = m0 + tc;
+      if (row2 < n && col2 < m) D2[(int64_t)row2 * P.ldd2 + col2] = cmul(P.s2, cconj(Cs[tc][tr]));
+    }
+  }
 }
 
 template <int TM, int TN, int kRole>
