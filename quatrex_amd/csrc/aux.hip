@@ -56,6 +56,32 @@ trace_kernel(const TraceJob* jobs, double2* out, int64_t sOut) {
   if (threadIdx.x == 0) out[(int64_t)e * sOut + J.slot] = cmk(J.sign * red[0][0], J.sign * red[1][0]);
 }
 
+// out[e*sOut + j] = ||X_j||_F for a list of matrices (rows x cols, ld).
+__global__ void __launch_bounds__(256)
+norm_kernel(const NormJob* jobs, double* out, int64_t sOut) {
+  const NormJob J = jobs[blockIdx.x];
+  const int e = blockIdx.y;
+  const double2* X = J.X + (int64_t)e * J.sX;
+  double acc = 0.0;
+  for (int idx = threadIdx.x; idx < J.rows * J.cols; idx += blockDim.x) {
+    const double2 v = X[(int64_t)(idx / J.cols) * J.ld + idx % J.cols];
+    acc += v.x * v.x + v.y * v.y;
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[(int64_t)e * sOut + blockIdx.x] = sqrt(red[0]);
+}
+
+__global__ void identity_kernel(double2* X, int n) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n * n) X[idx] = cmk((idx / n) == (idx % n) ? 1.0 : 0.0, 0.0);
+}
+
 __global__ void z2c_kernel(const double2* src, int64_t sSrc, float2* dst, int64_t sDst,
                            int64_t count) {
   const int e = blockIdx.y;
@@ -90,6 +116,18 @@ hipError_t launch_traces(const TraceJob* jobs_dev, int njobs, int nE, double2* o
                          hipStream_t stream) {
   if (njobs == 0 || nE == 0) return hipSuccess;
   hipLaunchKernelGGL(trace_kernel, dim3(njobs, nE), dim3(256), 0, stream, jobs_dev, out, sOut);
+  return hipGetLastError();
+}
+
+hipError_t launch_norms(const NormJob* jobs_dev, int njobs, int nE, double* out, int64_t sOut,
+                        hipStream_t stream) {
+  if (njobs == 0 || nE == 0) return hipSuccess;
+  hipLaunchKernelGGL(norm_kernel, dim3(njobs, nE), dim3(256), 0, stream, jobs_dev, out, sOut);
+  return hipGetLastError();
+}
+
+hipError_t launch_identity(double2* X, int n, hipStream_t stream) {
+  hipLaunchKernelGGL(identity_kernel, dim3((n * n + 255) / 256), dim3(256), 0, stream, X, n);
   return hipGetLastError();
 }
 

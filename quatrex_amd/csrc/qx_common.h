@@ -134,6 +134,8 @@ __device__ inline int xcd_remap(int w, int total) {
 // Host-side launchers (defined in the .hip translation units).
 hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream);
 hipError_t launch_zgemm_update(const ZGroup& g, int cfg, hipStream_t stream);
+// Same problems, products computed in complex64 on FP32 matrix cores.
+hipError_t launch_zgemm_f32(const ZGroup& g, int cfg, hipStream_t stream);
 hipError_t launch_cgemm(const CGroup& g, int cfg, hipStream_t stream);
 
 // Batched Gauss-Jordan inversion with partial pivoting, in place in S
@@ -158,6 +160,17 @@ struct TraceJob {
 };
 hipError_t launch_traces(const TraceJob* jobs_dev, int njobs, int nE, double2* out, int64_t sOut,
                          hipStream_t stream);
+
+// Frobenius norms of a list of matrices, per energy.
+struct NormJob {
+  const double2* X;
+  int64_t sX;
+  int32_t rows, cols, ld, pad_;
+};
+hipError_t launch_norms(const NormJob* jobs_dev, int njobs, int nE, double* out, int64_t sOut,
+                        hipStream_t stream);
+// n x n identity (one matrix; use energy stride 0 to share it).
+hipError_t launch_identity(double2* X, int n, hipStream_t stream);
 
 // Precision conversion helpers for the mixed path.
 hipError_t launch_z2c(const double2* src, int64_t sSrc, float2* dst, int64_t sDst, int64_t count,

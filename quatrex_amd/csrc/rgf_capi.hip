@@ -6,6 +6,7 @@
 // energies of a chunk advance through the same block step together, so every
 // launch covers (problems x energies x tiles) workgroups.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -248,7 +249,8 @@ int Ctx::gemm(const std::vector<ZProblem>& probs, int nE, int b) {
     hipEvent_t ev;
     int rc = prof_begin(&ev);
     if (rc) return rc;
-    rc = hip(launch_zgemm(g, tile_for(b), stream), "zgemm launch");
+    rc = hip(prec32 ? launch_zgemm_f32(g, tile_for(b), stream) : launch_zgemm(g, tile_for(b), stream),
+             "zgemm launch");
     if (rc) return rc;
     rc = prof_end(ev, group_flops(g), 0);
     if (rc) return rc;
@@ -287,20 +289,35 @@ struct Work {
   double2* stl = nullptr;    // [E][Dsz] Sigma~^< per block (current)
   double2* stg = nullptr;
   char* aux = nullptr;       // [E][aux_bytes]
+  // mixed precision only
+  double2* ident = nullptr;  // bmax x bmax identity, shared by all energies (stride 0)
+  double2* gru = nullptr;    // [E][Usz] off-diagonal G^R when the caller does not want it
+  double2* grl = nullptr;
+  double* norms = nullptr;   // [E][kMaxNormJobs]
   Mat slot(int q) const { return Mat{slots + q * slot_elems, sE, 0}; }
 };
 
+constexpr int kMaxNormJobs = 64;
+
 constexpr int kSlots = 11;
 
-size_t work_bytes(const Geo& g, int64_t E, bool solve, bool current, bool have_gr) {
+size_t work_bytes(const Geo& g, int64_t E, bool solve, bool current, bool have_gr,
+                  bool mixed = false, bool have_gr_off = true) {
   const size_t bb = (size_t)g.bmax * g.bmax * sizeof(double2);
   size_t per = (solve ? kSlots : 4) * bb + zinv_aux_bytes((int)g.bmax);
   if (!have_gr) per += g.Dsz * sizeof(double2);
   if (current) per += 2 * g.Dsz * sizeof(double2);
-  return per * (size_t)E + 256;
+  size_t fixed = 256;
+  if (mixed) {
+    fixed += bb + 256;
+    per += kMaxNormJobs * sizeof(double);
+    if (!have_gr_off) per += 2 * g.Usz * sizeof(double2);
+  }
+  return per * (size_t)E + fixed;
 }
 
-void carve(const Geo& g, int64_t E, bool solve, bool current, bool have_gr, char* base, Work* w) {
+void carve(const Geo& g, int64_t E, bool solve, bool current, bool have_gr, char* base, Work* w,
+           bool mixed = false, bool have_gr_off = true) {
   const int64_t bb = g.bmax * g.bmax;
   const int nsl = solve ? kSlots : 4;
   char* p = base;
@@ -319,28 +336,47 @@ void carve(const Geo& g, int64_t E, bool solve, bool current, bool have_gr, char
     p += (size_t)E * g.Dsz * sizeof(double2);
   }
   w->aux = p;
+  p += (size_t)E * zinv_aux_bytes((int)g.bmax);
+  if (mixed) {
+    p = base + (((size_t)(p - base) + 255) & ~(size_t)255);
+    w->ident = reinterpret_cast<double2*>(p);
+    p += (size_t)g.bmax * g.bmax * sizeof(double2);
+    p = base + (((size_t)(p - base) + 255) & ~(size_t)255);
+    w->norms = reinterpret_cast<double*>(p);
+    p += (size_t)E * kMaxNormJobs * sizeof(double);
+    if (!have_gr_off) {
+      w->gru = reinterpret_cast<double2*>(p);
+      p += (size_t)E * g.Usz * sizeof(double2);
+      w->grl = reinterpret_cast<double2*>(p);
+      p += (size_t)E * g.Usz * sizeof(double2);
+    }
+  }
 }
 
-// Energy-offset views of packed operands.
+// Energy-offset views of packed operands (each part has its own energy base:
+// caller arrays are indexed by global energy, workspace arrays by chunk).
 struct BTV {
   const Geo* g;
   double2* d;
   double2* u;
   double2* l;
   int64_t e0;
+  int64_t eu = -1, el = -1;  // -1: same as e0
+  int64_t bu() const { return eu < 0 ? e0 : eu; }
+  int64_t bl() const { return el < 0 ? e0 : el; }
   Mat D(int i) const {
     return d ? Mat{d + e0 * g->Dsz + g->od[i], g->Dsz, (int)g->bs[i]} : Mat{};
   }
   Mat U(int i) const {
-    return u ? Mat{u + e0 * g->Usz + g->ou[i], g->Usz, (int)g->bs[i + 1]} : Mat{};
+    return u ? Mat{u + bu() * g->Usz + g->ou[i], g->Usz, (int)g->bs[i + 1]} : Mat{};
   }
   Mat L(int i) const {
-    return l ? Mat{l + e0 * g->Usz + g->ou[i], g->Usz, (int)g->bs[i]} : Mat{};
+    return l ? Mat{l + bl() * g->Usz + g->ou[i], g->Usz, (int)g->bs[i]} : Mat{};
   }
 };
 
 BTV view(const Geo& g, const void* d, const void* u, const void* l, int64_t e0) {
-  return BTV{&g, (double2*)d, (double2*)u, (double2*)l, e0};
+  return BTV{&g, (double2*)d, (double2*)u, (double2*)l, e0, -1, -1};
 }
 
 Mat obc_mat(const void* const* arr, const Geo& g, int i, int64_t e0) {
@@ -427,7 +463,7 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
   const BTV& GL = io.GL;
   const BTV& GG = io.GG;
   BTV GR = io.GR;
-  if (!GR.d) GR = BTV{&g, w.gr, nullptr, nullptr, 0};
+  if (!GR.d) GR = BTV{&g, w.gr, GR.u, GR.l, 0, GR.u ? GR.bu() : -1, GR.l ? GR.bl() : -1};
   const void* const* obr = io.obc ? io.obc->retarded : nullptr;
   const void* const* obl = io.obc ? io.obc->lesser : nullptr;
   const void* const* obg = io.obc ? io.obc->greater : nullptr;
@@ -507,6 +543,33 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
     if (rc) return rc;
     rc = c->inv(S.p, S.s, bj, E, GR.D(j), w.aux, j + 1, e0);
     if (rc) return rc;
+    // Mixed precision: Newton steps g_j <- g_j + g_j (I - S64 g_j) with the Schur
+    // complement recomputed in FP64 from the stored g_i (products in FP64,
+    // the small correction product in FP32).
+    for (int it = 0; it < c->refine && c->prec32; ++it) {
+      Mat I = Mat{w.ident, 0, (int)g.bmax};
+      Mat P64 = sl(1, bi), S64 = sl(0, bj), Rj = sl(8, bj), Gt = sl(9, bj);
+      c->prec32 = false;
+      rc = c->gemm({PB(bj, bi).term(A.L(i), kOpN, GR.D(i), kOpN, bi).out(P64).done()}, E, bj);
+      if (!rc)
+        rc = c->gemm({PB(bj, bj)
+                          .term(P64, kOpN, A.U(i), kOpN, bi, MONE)
+                          .add(A.D(j), ONE)
+                          .add(OR(j), MONE)
+                          .out(S64)
+                          .done()},
+                     E, bj);
+      if (!rc)
+        rc = c->gemm({PB(bj, bj).term(S64, kOpN, GR.D(j), kOpN, bj, MONE).add(I, ONE).out(Rj).done()},
+                     E, bj);
+      c->prec32 = true;
+      if (rc) return rc;
+      rc = c->gemm({PB(bj, bj).term(GR.D(j), kOpN, Rj, kOpN, bj).add(GR.D(j), ONE).out(Gt).done()}, E,
+                   bj);
+      if (rc) return rc;
+      rc = c->gemm({PB(bj, bj).add(Gt, ONE).out(GR.D(j)).done()}, E, bj);
+      if (rc) return rc;
+    }
     // V = g_j W ; g^<>_j = V g_j^H   (V reuses the Y slots)
     Mat Vl = sl(4, bj), Vg = sl(5, bj);
     rc = c->gemm({PB(bj, bj).term(GR.D(j), kOpN, Wl, kOpN, bj).out(Vl).done(),
@@ -630,6 +693,127 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
 }
 
 }  // namespace
+
+// FP64 a-posteriori check of the mixed-precision result on a few diagonal
+// blocks i, through exact block-column identities that involve selected
+// blocks only (G^A_{k,i} = G^R_{i,k}^H, G^<_{i+1,i} = -G^<_{i,i+1}^H):
+//   (A~ G^R)_ii = I
+//   (A~ G^<)_ii = (S^< G^A)_ii,  S^< = Sigma^< + B^<   (and the same for >)
+// with (A~ X)_ii = A_{i,i-1} X_{i-1,i} + A~_ii X_ii + A_{i,i+1} X_{i+1,i}.
+// Relative residuals: ||R^R|| / sqrt(b_i) and
+// ||R^<|| / (||A_ii|| ||G^<_ii|| + ||S^<_ii|| ||G^R_ii||); the energy's value is
+// the maximum over the sampled blocks and the three functions.
+int check_chunk(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io, const BTV& GR,
+                const Work& w, std::vector<double>* resid) {
+  const int N = g.N;
+  const BTV& A = io.A;
+  const void* const* obr = io.obc ? io.obc->retarded : nullptr;
+  const void* const* obl = io.obc ? io.obc->lesser : nullptr;
+  const void* const* obg = io.obc ? io.obc->greater : nullptr;
+  std::vector<int> blocks = {0};
+  if (N > 2) blocks.push_back(N / 2);
+  if (N > 1) blocks.push_back(N - 1);
+  auto sl = [&](int q, int ld) { return with_ld(w.slot(q), ld); };
+  Mat I = Mat{w.ident, 0, (int)g.bmax};
+  std::vector<NormJob> jobs;
+  resid->assign(E, 0.0);
+  const int per_block = 9;
+  std::vector<double> h((size_t)E * kMaxNormJobs);
+  int rc;
+  for (size_t bi_ = 0; bi_ < blocks.size(); ++bi_) {
+    const int i = blocks[bi_];
+    const int b = (int)g.bs[i];
+    Mat OR = obc_mat(obr, g, i, e0);
+    // (A~ G^R)_ii - I
+    {
+      PB pa(b, b);
+      pa.term(A.D(i), kOpN, GR.D(i), kOpN, b);
+      if (i > 0) pa.term(A.L(i - 1), kOpN, GR.U(i - 1), kOpN, (int)g.bs[i - 1]);
+      PB pb(b, b);
+      if (i + 1 < N) pb.term(A.U(i), kOpN, GR.L(i), kOpN, (int)g.bs[i + 1]);
+      if (OR.p) pb.term(OR, kOpN, GR.D(i), kOpN, b, MONE);
+      pb.add(sl(0, b), ONE).add(I, MONE);
+      std::vector<ZProblem> l1 = {pa.out(sl(0, b)).done()}, l2 = {pb.out(sl(1, b)).done()},
+                            l3, l4;
+      // lesser and greater
+      const BTV* Xs[2] = {&io.GL, &io.GG};
+      const BTV* Ss[2] = {&io.SL, &io.SG};
+      const void* const* Bs[2] = {obl, obg};
+      for (int k = 0; k < 2; ++k) {
+        const BTV& X = *Xs[k];
+        const BTV& Sg = *Ss[k];
+        Mat OB = obc_mat(Bs[k], g, i, e0);
+        const int q = 2 + 4 * k;
+        PB pc(b, b);
+        pc.term(A.D(i), kOpN, X.D(i), kOpN, b);
+        if (i > 0) pc.term(A.L(i - 1), kOpN, X.U(i - 1), kOpN, (int)g.bs[i - 1]);
+        l1.push_back(pc.out(sl(q, b)).done());
+        PB pd(b, b);
+        if (i + 1 < N) pd.term(A.U(i), kOpN, X.U(i), kOpC, (int)g.bs[i + 1], MONE);
+        if (OR.p) pd.term(OR, kOpN, X.D(i), kOpN, b, MONE);
+        l2.push_back(pd.add(sl(q, b), ONE).out(sl(q + 1, b)).done());
+        PB pe(b, b);
+        pe.term(Sg.D(i), kOpN, GR.D(i), kOpC, b, MONE);
+        if (i > 0) pe.term(Sg.U(i - 1), kOpC, GR.L(i - 1), kOpC, (int)g.bs[i - 1]);
+        l3.push_back(pe.add(sl(q + 1, b), ONE).out(sl(q + 2, b)).done());
+        PB pf(b, b);
+        if (i + 1 < N) pf.term(Sg.U(i), kOpN, GR.U(i), kOpC, (int)g.bs[i + 1], MONE);
+        if (OB.p) pf.term(OB, kOpN, GR.D(i), kOpC, b, MONE);
+        l4.push_back(pf.add(sl(q + 2, b), ONE).out(sl(q + 3, b)).done());
+      }
+      for (auto* L : {&l1, &l2, &l3, &l4}) {
+        rc = c->gemm(*L, E, b);
+        if (rc) return rc;
+      }
+    }
+    // norms: residuals, then the scales
+    jobs.clear();
+    auto nj = [&](Mat m, int rows, int cols) {
+      NormJob j{};
+      j.X = m.p; j.sX = m.s; j.rows = rows; j.cols = cols; j.ld = m.ld;
+      jobs.push_back(j);
+    };
+    nj(sl(1, b), b, b);                 // R^R
+    nj(sl(5, b), b, b);                 // R^<
+    nj(sl(9, b), b, b);                 // R^>
+    nj(A.D(i), b, b);
+    nj(GR.D(i), b, b);
+    nj(io.GL.D(i), b, b);
+    nj(io.GG.D(i), b, b);
+    nj(io.SL.D(i), b, b);
+    nj(io.SG.D(i), b, b);
+    const size_t bytes = jobs.size() * sizeof(NormJob);
+    if (c->meta_used + bytes > c->meta_bytes) return c->fail(RGF_EHIP, "norm job area too small");
+    NormJob* dev = reinterpret_cast<NormJob*>(static_cast<char*>(c->meta) + c->meta_used);
+    rc = c->hip(hipMemcpyAsync(dev, jobs.data(), bytes, hipMemcpyHostToDevice, c->stream), "norm jobs");
+    if (rc) return rc;
+    rc = c->hip(hipStreamSynchronize(c->stream), "norm jobs sync");
+    if (rc) return rc;
+    rc = c->hip(launch_norms(dev, (int)jobs.size(), E, w.norms + bi_ * per_block, kMaxNormJobs,
+                             c->stream),
+                "norm launch");
+    if (rc) return rc;
+  }
+  rc = c->hip(hipMemcpyAsync(h.data(), w.norms, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream),
+              "read norms");
+  if (rc) return rc;
+  rc = c->hip(hipStreamSynchronize(c->stream), "read norms sync");
+  if (rc) return rc;
+  for (int e = 0; e < E; ++e) {
+    double worst = 0.0;
+    for (size_t bi_ = 0; bi_ < blocks.size(); ++bi_) {
+      const double* n = h.data() + (size_t)e * kMaxNormJobs + bi_ * per_block;
+      const double b = (double)g.bs[blocks[bi_]];
+      const double rr = n[0] / std::sqrt(b);
+      const double rl = n[1] / std::max(n[3] * n[5] + n[7] * n[4], 1e-300);
+      const double rg = n[2] / std::max(n[3] * n[6] + n[8] * n[4], 1e-300);
+      worst = std::max({worst, rr, rl, rg});
+    }
+    (*resid)[e] = worst;
+  }
+  return RGF_OK;
+}
 
 namespace qx {
 
@@ -934,7 +1118,7 @@ int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energ
     io.SG = view(g, sigma_greater->diag, sigma_greater->upper, nullptr, e0);
     io.obc = obc;
     io.GR = have_gr ? view(g, g_retarded->diag, g_retarded->upper, g_retarded->lower, e0)
-                    : BTV{&g, nullptr, nullptr, nullptr, 0};
+                    : BTV{&g, nullptr, nullptr, nullptr, 0, -1, -1};
     io.GL = view(g, g_lesser->diag, g_lesser->upper, g_lesser->lower, e0);
     io.GG = view(g, g_greater->diag, g_greater->upper, g_greater->lower, e0);
     io.current = static_cast<double2*>(current);
@@ -969,9 +1153,122 @@ int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_e
                              const rgf_mixed_opts* opts, rgf_bt* g_retarded, rgf_bt* g_lesser,
                              rgf_bt* g_greater, void* current) {
   if (!ctx) return RGF_EINVAL;
-  (void)geom; (void)n_energies; (void)a; (void)sigma_lesser; (void)sigma_greater; (void)obc;
-  (void)opts; (void)g_retarded; (void)g_lesser; (void)g_greater; (void)current;
-  return ctx->fail(RGF_EUNSUPPORTED, "mixed precision path not built yet");
+  GUARD_BEGIN
+  qx::Geo g;
+  int rc = g.init(geom, &ctx->last_error);
+  if (rc) return rc;
+  if (!opts) return ctx->fail(RGF_EINVAL, "mixed: options required");
+  if (opts->refine < 0) return ctx->fail(RGF_EINVAL, "mixed: refine must be >= 0");
+  if (n_energies < 0) return ctx->fail(RGF_EINVAL, "n_energies < 0");
+  if (n_energies == 0) return RGF_OK;
+  const bool multi = g.N > 1;
+  if (!a || !a->diag || (multi && (!a->upper || !a->lower)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: A needs diag, upper and lower");
+  if (!sigma_lesser || !sigma_greater || !sigma_lesser->diag || !sigma_greater->diag ||
+      (multi && (!sigma_lesser->upper || !sigma_greater->upper)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: Sigma needs diag and upper");
+  if (!g_lesser || !g_greater || !g_lesser->diag || !g_greater->diag ||
+      (multi && (!g_lesser->upper || !g_greater->upper)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: G^< and G^> need diag and upper outputs");
+  if (current) {
+    const int last = g.N - 1;
+    if (!obc || !obc->lesser || !obc->greater || !obc->lesser[0] || !obc->greater[0] ||
+        !obc->lesser[last] || !obc->greater[last])
+      return ctx->fail(RGF_EINVAL,
+                       "current needs lesser/greater OBC blocks on the first and last block");
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  const bool have_gr = g_retarded && g_retarded->diag;
+  const bool have_off = have_gr && (!multi || (g_retarded->upper && g_retarded->lower));
+  const bool cur = current != nullptr;
+  const size_t per = work_bytes(g, 1, true, cur, have_gr, true, have_off);
+  int64_t chunk = ctx->chunk_for(g, n_energies, true, cur, have_gr);
+  chunk = std::max<int64_t>(1, std::min<int64_t>(chunk, (int64_t)(ctx->ws_budget / per)));
+  rc = ctx->upload_geometry(g);
+  if (rc) return rc;
+  char* ws = nullptr;
+  rc = ctx->ensure_workspace(work_bytes(g, chunk, true, cur, have_gr, true, have_off), &ws);
+  if (rc) return rc;
+  rc = ctx->reset_info();
+  if (rc) return rc;
+  Work w;
+  carve(g, chunk, true, cur, have_gr, ws, &w, true, have_off);
+  rc = ctx->hip(launch_identity(w.ident, (int)g.bmax, ctx->stream), "identity");
+  if (rc) return rc;
+  std::vector<double> resid;
+  std::vector<int32_t> flags;
+  for (int64_t e0 = 0; e0 < n_energies; e0 += chunk) {
+    const int E = (int)std::min<int64_t>(chunk, n_energies - e0);
+    SolveIO io;
+    io.A = view(g, a->diag, a->upper, a->lower, e0);
+    io.SL = view(g, sigma_lesser->diag, sigma_lesser->upper, nullptr, e0);
+    io.SG = view(g, sigma_greater->diag, sigma_greater->upper, nullptr, e0);
+    io.obc = obc;
+    if (have_gr) {
+      io.GR = view(g, g_retarded->diag, have_off ? g_retarded->upper : w.gru,
+                   have_off ? g_retarded->lower : w.grl, e0);
+      if (!have_off) { io.GR.eu = 0; io.GR.el = 0; }
+    } else {
+      io.GR = BTV{&g, nullptr, w.gru, w.grl, 0, 0, 0};
+    }
+    io.GL = view(g, g_lesser->diag, g_lesser->upper, g_lesser->lower, e0);
+    io.GG = view(g, g_greater->diag, g_greater->upper, g_greater->lower, e0);
+    io.current = static_cast<double2*>(current);
+    ctx->prec32 = true;
+    ctx->refine = opts->refine;
+    rc = sweep_chunk_solve(ctx, g, e0, E, io, w);
+    ctx->prec32 = false;
+    ctx->refine = 0;
+    if (rc) return rc;
+    const bool want_check = opts->fallback_tol > 0 || opts->residuals || opts->fallback_flags;
+    if (!want_check) continue;
+    BTV GRc = io.GR;
+    if (!GRc.d) GRc = BTV{&g, w.gr, io.GR.u, io.GR.l, 0, io.GR.bu(), io.GR.bl()};
+    rc = check_chunk(ctx, g, e0, E, io, GRc, w, &resid);
+    if (rc) return rc;
+    flags.assign(E, 0);
+    std::vector<int> redo;
+    if (opts->fallback_tol > 0)
+      for (int e = 0; e < E; ++e)
+        if (!(resid[e] <= opts->fallback_tol)) { flags[e] = 1; redo.push_back(e); }
+    if (!redo.empty()) {
+      if ((int)redo.size() * 4 > E) {
+        rc = sweep_chunk_solve(ctx, g, e0, E, io, w);
+        if (rc) return rc;
+      } else {
+        for (int e : redo) {
+          SolveIO one = io;
+          const int64_t ge = e0 + e;
+          one.A = view(g, a->diag, a->upper, a->lower, ge);
+          one.SL = view(g, sigma_lesser->diag, sigma_lesser->upper, nullptr, ge);
+          one.SG = view(g, sigma_greater->diag, sigma_greater->upper, nullptr, ge);
+          one.GL = view(g, g_lesser->diag, g_lesser->upper, g_lesser->lower, ge);
+          one.GG = view(g, g_greater->diag, g_greater->upper, g_greater->lower, ge);
+          one.GR.e0 = have_gr ? ge : 0;
+          if (!have_off) { one.GR.eu = 0; one.GR.el = 0; }
+          else if (have_gr) { one.GR.eu = -1; one.GR.el = -1; }
+          rc = sweep_chunk_solve(ctx, g, ge, 1, one, w);
+          if (rc) return rc;
+        }
+      }
+    }
+    if (opts->residuals) {
+      rc = ctx->hip(hipMemcpyAsync(opts->residuals + e0, resid.data(), E * sizeof(double),
+                                   hipMemcpyHostToDevice, ctx->stream),
+                    "write residuals");
+      if (rc) return rc;
+    }
+    if (opts->fallback_flags) {
+      rc = ctx->hip(hipMemcpyAsync(opts->fallback_flags + e0, flags.data(), E * sizeof(int32_t),
+                                   hipMemcpyHostToDevice, ctx->stream),
+                    "write flags");
+      if (rc) return rc;
+    }
+    rc = ctx->hip(hipStreamSynchronize(ctx->stream), "mixed sync");  // host vectors reused
+    if (rc) return rc;
+  }
+  return RGF_OK;
+  GUARD_END
 }
 
 }  // extern "C"

@@ -53,6 +53,11 @@ struct Ctx {
   int* sizes_dev = nullptr;
   std::string last_error;
 
+  // mixed precision: sweep products on FP32 matrix cores; `refine` FP64
+  // Newton steps per left-connected inverse
+  bool prec32 = false;
+  int refine = 0;
+
   // launch profiling (RGF_OPT_PROFILE)
   bool profile = false;
   struct Timed { hipEvent_t a, b; double flops; int kind; };

@@ -10,6 +10,8 @@
 // same for every op combination.  Global loads are issued for the next k-tile
 // before the MFMAs of the current one (register double buffering, two LDS
 // stages, one barrier per k-tile).
+#include <type_traits>
+
 #include "qx_common.h"
 
 namespace qx {
@@ -49,9 +51,12 @@ __device__ inline void load_a(const ZTerm& t, int e, int m, int m0, int k0, int 
   }
 }
 
-template <int TM, int TN>
+__device__ inline void put(double2* p, double2 v) { *p = v; }
+__device__ inline void put(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
+template <int TM, int TN, typename LT>
 __device__ inline void store_a(const ZTerm& t, int tid, const double2 (&ra)[TileShape<TM, TN>::kAPer],
-                               double2 (*As)[TM + PAD]) {
+                               LT (*As)[TM + PAD]) {
   using S = TileShape<TM, TN>;
   const bool cj = (t.opA == kOpC);
 #pragma unroll
@@ -62,7 +67,7 @@ __device__ inline void store_a(const ZTerm& t, int tid, const double2 (&ra)[Tile
     else { mm = idx % TM; kk = idx / TM; }
     double2 v = ra[i];
     if (cj) v.y = -v.y;
-    As[kk][mm] = cmul(t.alpha, v);
+    put(&As[kk][mm], cmul(t.alpha, v));
   }
 }
 
@@ -87,9 +92,9 @@ __device__ inline void load_b(const ZTerm& t, int e, int n, int n0, int k0, int 
   }
 }
 
-template <int TM, int TN>
+template <int TM, int TN, typename LT>
 __device__ inline void store_b(const ZTerm& t, int tid, const double2 (&rb)[TileShape<TM, TN>::kBPer],
-                               double2 (*Bs)[TN + PAD]) {
+                               LT (*Bs)[TN + PAD]) {
   using S = TileShape<TM, TN>;
   const bool cj = (t.opB == kOpC);
 #pragma unroll
@@ -100,19 +105,17 @@ __device__ inline void store_b(const ZTerm& t, int tid, const double2 (&rb)[Tile
     else { kk = idx % TK; nn = idx / TK; }
     double2 v = rb[i];
     if (cj) v.y = -v.y;
-    Bs[kk][nn] = v;
+    put(&Bs[kk][nn], v);
   }
 }
 
-template <int TM, int TN>
+template <int TM, int TN, typename LT>
 struct Smem {
-  static constexpr int kA = TK * (TM + PAD);  // one LDS stage of op(A), k-major
+  static constexpr int kA = TK * (TM + PAD);  // one LDS stage of op(A), k-major (LT elements)
   static constexpr int kB = TK * (TN + PAD);  // one LDS stage of op(B), k-major
-  static constexpr int kMain = 2 * kA + 2 * kB;
-  static constexpr int kTile = TM * (TN + 1);
-  static constexpr int kTileT = TN * (TM + 1);
-  static constexpr int kSize = kMain > kTile ? (kMain > kTileT ? kMain : kTileT)
-                                             : (kTile > kTileT ? kTile : kTileT);
+  static constexpr int kMainBytes = (2 * kA + 2 * kB) * (int)sizeof(LT);
+  static constexpr int kTileBytes = TM * (TN + 1) * (int)sizeof(double2);
+  static constexpr int kBytes = kMainBytes > kTileBytes ? kMainBytes : kTileBytes;
 };
 
 // f64 MFMA with the BLGP field used as the negate-A modifier.
@@ -125,15 +128,21 @@ __device__ inline double4_t mfma_nega(double a, double b, double4_t c) {
 
 // kRole only separates kernel names in traces: 0 = sweep products,
 // 1 = trailing updates inside the blocked inversion.
-template <int TM, int TN, int kRole>
+// kF32: operands are rounded to complex64 while staged into LDS and the
+// products run on v_mfma_f32_16x16x4f32 with FP32 accumulation; loads,
+// addends and stores stay complex128 (the mixed-precision sweeps).
+template <int TM, int TN, int kRole, bool kF32>
 __global__ void __launch_bounds__((TileShape<TM, TN>::kThreads), (TM * TN > 4096 ? 1 : 2))
 zgemm_grouped_kernel(const ZGroup g) {
   using S = TileShape<TM, TN>;
-  using L = Smem<TM, TN>;
-  __shared__ double2 smem[L::kSize];
-  auto As = [&](int buf) { return reinterpret_cast<double2(*)[TM + PAD]>(smem + buf * L::kA); };
+  using LT = typename std::conditional<kF32, float2, double2>::type;
+  using AccT = typename std::conditional<kF32, float4_t, double4_t>::type;
+  using L = Smem<TM, TN, LT>;
+  __shared__ __align__(16) char smem_raw[L::kBytes];
+  LT* smem = reinterpret_cast<LT*>(smem_raw);
+  auto As = [&](int buf) { return reinterpret_cast<LT(*)[TM + PAD]>(smem + buf * L::kA); };
   auto Bs = [&](int buf) {
-    return reinterpret_cast<double2(*)[TN + PAD]>(smem + 2 * L::kA + buf * L::kB);
+    return reinterpret_cast<LT(*)[TN + PAD]>(smem + 2 * L::kA + buf * L::kB);
   };
 
   const int total = g.wg_base[g.nprob];
@@ -154,13 +163,13 @@ zgemm_grouped_kernel(const ZGroup g) {
   const int wm = wave / S::kWavesN;
   const int wn = wave % S::kWavesN;
 
-  double4_t accr[2][2], acci[2][2];
+  AccT accr[2][2], acci[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      accr[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
-      acci[i][j] = double4_t{0.0, 0.0, 0.0, 0.0};
+      accr[i][j] = AccT{0, 0, 0, 0};
+      acci[i][j] = AccT{0, 0, 0, 0};
     }
 
   for (int ti = 0; ti < P.nterms; ++ti) {
@@ -169,8 +178,8 @@ zgemm_grouped_kernel(const ZGroup g) {
     double2 ra[S::kAPer], rb[S::kBPer];
     load_a<TM, TN>(T, e, m, m0, 0, tid, ra);
     load_b<TM, TN>(T, e, n, n0, 0, tid, rb);
-    store_a<TM, TN>(T, tid, ra, As(0));
-    store_b<TM, TN>(T, tid, rb, Bs(0));
+    store_a<TM, TN, LT>(T, tid, ra, As(0));
+    store_b<TM, TN, LT>(T, tid, rb, Bs(0));
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
       const int cur = kt & 1;
@@ -182,7 +191,7 @@ zgemm_grouped_kernel(const ZGroup g) {
       auto as = As(cur);
       auto bs = Bs(cur);
       // fragments double-buffered across the k-steps of this LDS stage
-      double2 fa[2][2], fb[2][2];
+      LT fa[2][2], fb[2][2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) fa[0][i] = as[lane >> 4][wm * 32 + i * 16 + (lane & 15)];
 #pragma unroll
@@ -197,26 +206,52 @@ zgemm_grouped_kernel(const ZGroup g) {
 #pragma unroll
           for (int j = 0; j < 2; ++j) fb[c ^ 1][j] = bs[kk][wn * 32 + j * 16 + (lane & 15)];
         }
+        if constexpr (kF32) {
+          float nai[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+          for (int i = 0; i < 2; ++i) nai[i] = -fa[c][i].y;
 #pragma unroll
-          for (int j = 0; j < 2; ++j) accr[i][j] = mfma(fa[c][i].x, fb[c][j].x, accr[i][j]);
+          for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+              accr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[c][i].x, fb[c][j].x, accr[i][j], 0, 0, 0);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) acci[i][j] = mfma(fa[c][i].x, fb[c][j].y, acci[i][j]);
+          for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+              acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[c][i].x, fb[c][j].y, acci[i][j], 0, 0, 0);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) accr[i][j] = mfma_nega(fa[c][i].y, fb[c][j].y, accr[i][j]);
+          for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+              accr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[i], fb[c][j].y, accr[i][j], 0, 0, 0);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) acci[i][j] = mfma(fa[c][i].y, fb[c][j].x, acci[i][j]);
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+              acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[c][i].y, fb[c][j].x, acci[i][j], 0, 0, 0);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) accr[i][j] = mfma(fa[c][i].x, fb[c][j].x, accr[i][j]);
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acci[i][j] = mfma(fa[c][i].x, fb[c][j].y, acci[i][j]);
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) accr[i][j] = mfma_nega(fa[c][i].y, fb[c][j].y, accr[i][j]);
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acci[i][j] = mfma(fa[c][i].y, fb[c][j].x, acci[i][j]);
+        }
       }
       if (more) {
-        store_a<TM, TN>(T, tid, ra, As(cur ^ 1));
-        store_b<TM, TN>(T, tid, rb, Bs(cur ^ 1));
+        store_a<TM, TN, LT>(T, tid, ra, As(cur ^ 1));
+        store_b<TM, TN, LT>(T, tid, rb, Bs(cur ^ 1));
       }
       __syncthreads();
     }
@@ -225,16 +260,19 @@ zgemm_grouped_kernel(const ZGroup g) {
   // ---- epilogue, staged through LDS so global accesses are row-coalesced ----
   // For v_mfma_f64_16x16x4f64 register r of a lane holds D[4*r + lane/16][lane%16]
   // of each 16x16 sub-tile (probed on gfx950).
+  // (v_mfma_f32_16x16x4f32: register r of lane l holds D[4*(l/16) + r][l%16].)
   constexpr int kPer = TM * TN / S::kThreads;
-  auto Cs = reinterpret_cast<double2(*)[TN + 1]>(smem);
+  auto Cs = reinterpret_cast<double2(*)[TN + 1]>(smem_raw);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        Cs[wm * 32 + i * 16 + r * 4 + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)] =
-            cmk(accr[i][j][r], acci[i][j][r]);
+      for (int r = 0; r < 4; ++r) {
+        const int rr = kF32 ? ((lane >> 4) * 4 + r) : (r * 4 + (lane >> 4));
+        Cs[wm * 32 + i * 16 + rr][wn * 32 + j * 16 + (lane & 15)] =
+            cmk((double)accr[i][j][r], (double)acci[i][j][r]);
+      }
   __syncthreads();
   const int64_t eo = (int64_t)e;
   double2* D = P.D + eo * P.sD;
@@ -297,7 +335,7 @@ zgemm_grouped_kernel(const ZGroup g) {
   }
 }
 
-template <int TM, int TN, int kRole>
+template <int TM, int TN, int kRole, bool kF32 = false>
 hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   ZGroup g = g0;
   int base = 0;
@@ -310,7 +348,7 @@ hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   }
   g.wg_base[g.nprob] = base;
   if (base == 0) return hipSuccess;
-  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole>), dim3(base), dim3(TileShape<TM, TN>::kThreads),
+  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole, kF32>), dim3(base), dim3(TileShape<TM, TN>::kThreads),
                      0, stream, g);
   return hipGetLastError();
 }
@@ -342,6 +380,13 @@ hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream) {
     case kTile64x32: return launch_cfg<64, 32, 0>(g, stream);
     case kTile128x64: return launch_cfg<128, 64, 0>(g, stream);
     default: return launch_cfg<64, 64, 0>(g, stream);
+  }
+}
+
+hipError_t launch_zgemm_f32(const ZGroup& g, int cfg, hipStream_t stream) {
+  switch (cfg) {
+    case kTile32x32: return launch_cfg<32, 32, 0, true>(g, stream);
+    default: return launch_cfg<64, 64, 0, true>(g, stream);
   }
 }
 

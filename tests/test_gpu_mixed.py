@@ -1,0 +1,131 @@
+"""Mixed-precision selected solve (complex64 products on FP32 matrix cores,
+FP64 inversions, FP64 a-posteriori check with FP64 fallback)."""
+
+import numpy as np
+import pytest
+
+from oracle.dense import dense_solve
+from quatrex_amd import solve as S
+from quatrex_amd import workloads as W
+from quatrex_amd.layout import Geometry
+from tests.helpers import block_rel_errors, tridiagonal_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(device, nb, b, ne, eta, seed):
+    import torch
+
+    g = Geometry.uniform(nb, b)
+    model = W.make_device(g, seed)
+    en = W.draw_energies(ne)
+    A = W.system_matrix(model, en, eta)
+    sld, slu, sgd, sgu = W.sigmas(model, ne)
+    t = lambda x: torch.as_tensor(np.ascontiguousarray(x), device=device)  # noqa: E731
+    return g, A, (sld, slu), (sgd, sgu), t
+
+
+def _run(device, g, A, SL, SG, t, precision, mixed=None):
+    E = A[0].shape[0]
+    GR, GL, GG = (S.packed_empty(g, E, device) for _ in range(3))
+    S.selected_solve(g, tuple(map(t, A)), tuple(map(t, SL)), tuple(map(t, SG)), g_lesser=GL,
+                     g_greater=GG, g_retarded=GR, precision=precision, mixed=mixed)
+    S.default_context(device).status()
+    return [tuple(x.cpu().numpy() for x in X) for X in (GR, GL, GG)]
+
+
+def _errors(g, outs, A, SL, SG):
+    E = A[0].shape[0]
+    z = np.zeros_like(SL[1])
+    ref = dense_solve(g.to_dense(*A), g.to_dense(SL[0], SL[1], z), g.to_dense(SG[0], SG[1], z))
+    bs = list(g.block_sizes)
+    m = tridiagonal_mask(bs)
+    return [block_rel_errors(g.to_dense(*X), r * m, bs) for X, r in zip(outs, ref)]
+
+
+def _cpu_residual(g, A, SL, SG, outs):
+    """Same identities as the library's check, evaluated in numpy (block 0, N/2, N-1)."""
+    GR, GL, GG = outs
+    N = g.n_blocks
+    E = A[0].shape[0]
+    worst = np.zeros(E)
+    for i in sorted({0, N // 2, N - 1}):
+        b = g.block_sizes[i]
+        D = lambda x, k: g.diag(x, k)  # noqa: E731
+        U = lambda x, k: g.upper(x, k)  # noqa: E731
+        L = lambda x, k: g.lower(x, k)  # noqa: E731
+        H = lambda x: np.conj(np.swapaxes(x, -1, -2))  # noqa: E731
+        rr = D(A[0], i) @ D(GR[0], i) - np.eye(b)
+        if i > 0:
+            rr = rr + L(A[2], i - 1) @ U(GR[1], i - 1)
+        if i < N - 1:
+            rr = rr + U(A[1], i) @ L(GR[2], i)
+        res = [np.linalg.norm(rr, axis=(-2, -1)) / np.sqrt(b)]
+        for X, Sg in ((GL, SL), (GG, SG)):
+            r = D(A[0], i) @ D(X[0], i) - D(Sg[0], i) @ H(D(GR[0], i))
+            if i > 0:
+                r = r + L(A[2], i - 1) @ U(X[1], i - 1) + H(U(Sg[1], i - 1)) @ H(L(GR[2], i - 1))
+            if i < N - 1:
+                r = r - U(A[1], i) @ H(U(X[1], i)) - U(Sg[1], i) @ H(U(GR[1], i))
+            scale = (np.linalg.norm(D(A[0], i), axis=(-2, -1)) * np.linalg.norm(D(X[0], i), axis=(-2, -1))
+                     + np.linalg.norm(D(Sg[0], i), axis=(-2, -1)) * np.linalg.norm(D(GR[0], i), axis=(-2, -1)))
+            res.append(np.linalg.norm(r, axis=(-2, -1)) / scale)
+        worst = np.maximum(worst, np.max(res, axis=0))
+    return worst
+
+
+@pytest.mark.parametrize("eta", [1e-2, 1e-3])
+def test_mixed_error_is_fp32_level(device, eta):
+    g, A, SL, SG, t = _case(device, 6, 64, 3, eta, 3001)
+    outs = _run(device, g, A, SL, SG, t, "mixed", S.MixedOptions(refine=1, fallback_tol=0.0))
+    errs = _errors(g, outs, A, SL, SG)
+    assert max(errs) < 1e-2, errs
+    assert max(errs) > 1e-12, errs  # it really ran in reduced precision
+
+
+def test_mixed_residual_matches_cpu_and_flags_fallback(device):
+    import torch
+
+    g, A, SL, SG, t = _case(device, 8, 48, 4, 1e-3, 3002)
+    E = 4
+    res = torch.zeros(E, dtype=torch.float64, device=device)
+    flags = torch.zeros(E, dtype=torch.int32, device=device)
+    # no fallback: residuals of the mixed result, checked against numpy
+    outs = _run(device, g, A, SL, SG, t, "mixed",
+                S.MixedOptions(refine=1, fallback_tol=0.0, residuals=res, fallback_flags=flags))
+    cpu = _cpu_residual(g, A, SL, SG, outs)
+    gpu = res.cpu().numpy()
+    assert np.allclose(gpu, cpu, rtol=1e-6, atol=1e-15), (gpu, cpu)
+    assert np.all(gpu > 1e-9)
+    assert flags.sum().item() == 0
+    # exact (FP64) results satisfy the identities to rounding
+    exact = _run(device, g, A, SL, SG, t, "fp64")
+    assert np.all(_cpu_residual(g, A, SL, SG, exact) < 1e-12)
+    # with the 1e-8 tolerance every energy falls back to FP64
+    outs = _run(device, g, A, SL, SG, t, "mixed",
+                S.MixedOptions(refine=1, fallback_tol=1e-8, residuals=res, fallback_flags=flags))
+    assert flags.cpu().numpy().tolist() == [1] * E
+    assert max(_errors(g, outs, A, SL, SG)) <= 1e-10
+
+
+def test_mixed_single_energy_fallback_path(device):
+    """Few failing energies take the per-energy FP64 path; the rest stay mixed."""
+    import torch
+
+    g, A, SL, SG, t = _case(device, 5, 32, 8, 1e-2, 3003)
+    E = 8
+    res = torch.zeros(E, dtype=torch.float64, device=device)
+    flags = torch.zeros(E, dtype=torch.int32, device=device)
+    outs = _run(device, g, A, SL, SG, t, "mixed",
+                S.MixedOptions(refine=0, fallback_tol=0.0, residuals=res, fallback_flags=flags))
+    r = res.cpu().numpy()
+    tol = 0.5 * float(np.sort(r)[-2] + np.sort(r)[-1])  # only the worst energy exceeds it
+    outs = _run(device, g, A, SL, SG, t, "mixed",
+                S.MixedOptions(refine=0, fallback_tol=tol, residuals=res, fallback_flags=flags))
+    f = flags.cpu().numpy()
+    assert f.sum() == 1 and f[np.argmax(r)] == 1
+    worst = int(np.argmax(r))
+    one = [tuple(p[worst:worst + 1] for p in X) for X in outs]
+    errs = _errors(g, one, tuple(x[worst:worst + 1] for x in A), tuple(x[worst:worst + 1] for x in SL),
+                   tuple(x[worst:worst + 1] for x in SG))
+    assert max(errs) <= 1e-10
