@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--chunk", type=int, default=0, help="max energies per internal sweep (0=auto)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
+    ap.add_argument("--plain-sample", type=int, default=8,
+                    help="energies for the one-energy-at-a-time unfused FP64 reference point")
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "zgemm_traffic_32x256.json"))
     return ap.parse_args()
 
@@ -173,6 +176,96 @@ def main():
         except Exception:
             traffic = None
 
+    extra = {}
+    if not args.no_mixed:
+        # FP64 results of a sample of energies are the reference for the mixed errors
+        ns = min(16, E)
+        ref = [X[0][:ns].clone() for X in (GR, GL, GG)]
+
+        def rel_err(X, R):
+            d = (X[0][:ns] - R).reshape(ns, g.n_blocks, -1)
+            r = R.reshape(ns, g.n_blocks, -1)
+            return float((torch.linalg.norm(d, dim=-1) / torch.linalg.norm(r, dim=-1)).max())
+
+        mixed = S.MixedOptions(refine=0, fallback_tol=0.0)
+
+        def mstep():
+            S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
+                             ctx=ctx, precision="mixed", mixed=mixed)
+
+        mstep()
+        ctx.status()
+        torch.cuda.synchronize()
+        tm0 = time.perf_counter()
+        for _ in range(args.steps):
+            mstep()
+        torch.cuda.synchronize()
+        tm = time.perf_counter() - tm0
+        ctx.status()
+        eps_mixed = E * args.steps / tm
+        errs = {k: rel_err(X, R) for k, X, R in zip(("G^R", "G^<", "G^>"), (GR, GL, GG), ref)}
+        # mixed + FP64 a-posteriori check, FP64 fallback above 1e-8
+        flags = torch.zeros(E, dtype=torch.int32, device=dev)
+        resid = torch.zeros(E, dtype=torch.float64, device=dev)
+        checked = S.MixedOptions(refine=1, fallback_tol=1e-8, fallback_flags=flags, residuals=resid)
+        torch.cuda.synchronize()
+        tc0 = time.perf_counter()
+        S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
+                         ctx=ctx, precision="mixed", mixed=checked)
+        torch.cuda.synchronize()
+        tc = time.perf_counter() - tc0
+        ctx.status()
+        errs_fb = {k: rel_err(X, R) for k, X, R in zip(("G^R", "G^<", "G^>"), (GR, GL, GG), ref)}
+        # time-weighted roofline: F_gemm at the FP32 matrix peak, F_inv at the FP64 peak
+        b3 = 8.0 * args.block_size ** 3
+        t_roof = b3 * ((33 * args.n_blocks - 29) / (FP32_PEAK_TFLOPS * 1e12)
+                       + args.n_blocks / (FP64_PEAK_TFLOPS * 1e12))
+        extra["mixed"] = {
+            "value": round(eps_mixed, 3),
+            "unit": "energy_points/s",
+            "speedup_vs_fp64": round(eps_mixed / eps, 3),
+            "max_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+            "error_sample_energies": ns,
+            "roofline_frac": round(t_roof * eps_mixed, 4),
+            "with_check_and_fp64_fallback": {
+                "value": round(E / tc, 3),
+                "fallback_fraction": round(float(flags.float().mean()), 4),
+                "median_residual": float(f"{float(resid.median()):.3e}"),
+                "max_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs_fb.items()},
+            },
+        }
+    if args.plain_sample > 0:
+        # plain FP64 GPU RGF: one energy per sweep, one product group per launch
+        ns = min(args.plain_sample, E)
+        sl_ = slice(0, ns)
+        view = lambda T: tuple(t[sl_] if t is not None else None for t in T)  # noqa: E731
+        pctx = S.Context(dev)
+        pctx.set_option(1, 1)
+        pctx.set_option(5, 1)
+
+        def pstep():
+            S.selected_solve(g, view(A), (sld[sl_], slu[sl_]), (sgd[sl_], sgu[sl_]), g_lesser=view(GL),
+                             g_greater=view(GG), g_retarded=view(GR), ctx=pctx)
+
+        pstep()
+        pctx.status()
+        tp0 = time.perf_counter()
+        pstep()
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - tp0
+        pctx.status()
+        eps_plain = ns / tp
+        extra["plain_fp64"] = {
+            "value": round(eps_plain, 3),
+            "unit": "energy_points/s",
+            "definition": "one energy per sweep, one product group per launch (FP64)",
+            "sample_energies": ns,
+            "batched_fp64_speedup": round(eps / eps_plain, 2),
+        }
+        if "mixed" in extra:
+            extra["mixed"]["speedup_vs_plain_fp64"] = round(extra["mixed"]["value"] / eps_plain, 2)
+        pctx.close()
+
     out = {
         "metric": "RGF energy-points/s at 32x256 c128 (FP64 vs mixed); p50/p99 solve latency",
         "value": round(eps, 3),
@@ -215,7 +308,9 @@ def main():
             "inverse_ms_share": round(prof["inv_ms"] / max(prof["inv_ms"] + prof["gemm_ms"], 1e-9), 4),
             "inverse_tflops": round(prof["inv_flops"] / max(prof["inv_ms"], 1e-9) / 1e9, 3),
         },
+        **extra,
     }
+
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model, energies, args.eta, args.cpu_seconds)
     print(json.dumps(out), flush=True)

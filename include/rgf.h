@@ -47,6 +47,9 @@ extern "C" {
 #define RGF_OPT_MAX_CHUNK 1        /* max energies per internal sweep (0 = auto)           */
 #define RGF_OPT_WORKSPACE_BUDGET 2 /* bytes the context may allocate for workspace (auto)  */
 #define RGF_OPT_TILE_CFG 3         /* GEMM tile config for all block sizes (-1 = autotune) */
+#define RGF_OPT_UNFUSED 5          /* 1: launch every product group on its own (the plain,
+                                      ungrouped op sequence; with RGF_OPT_MAX_CHUNK = 1 this is
+                                      the one-energy-at-a-time baseline)                      */
 #define RGF_OPT_PROFILE 4          /* 1: time every sweep GEMM launch and every block inversion
                                       with HIP events on the context stream (0: off)          */
 

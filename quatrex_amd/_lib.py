@@ -24,6 +24,7 @@ RGF_OPT_MAX_CHUNK = 1
 RGF_OPT_WORKSPACE_BUDGET = 2
 RGF_OPT_TILE_CFG = 3
 RGF_OPT_PROFILE = 4
+RGF_OPT_UNFUSED = 5
 RGF_OPT_TILE_FOR_BLOCK = 1000  # + block size
 
 # Every function declared in include/rgf.h (checked by the tests).

@@ -240,7 +240,8 @@ int Ctx::gemm(const std::vector<ZProblem>& probs, int nE, int b) {
     std::memset(&g, 0, sizeof(g));
     g.nE = nE;
     int np = 0;
-    while (i < probs.size() && np < kMaxProblems) {
+    const int cap = unfused ? 1 : kMaxProblems;
+    while (i < probs.size() && np < cap) {
       if (probs[i].D) g.p[np++] = probs[i];
       ++i;
     }
@@ -937,6 +938,7 @@ int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value) {
       ctx->ws_budget = (size_t)value;
       return RGF_OK;
     case RGF_OPT_PROFILE: ctx->profile = value != 0; return RGF_OK;
+    case RGF_OPT_UNFUSED: ctx->unfused = value != 0; return RGF_OK;
     case RGF_OPT_TILE_CFG:
       if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
       ctx->tile_override = (int)value;

@@ -57,6 +57,8 @@ struct Ctx {
   // Newton steps per left-connected inverse
   bool prec32 = false;
   int refine = 0;
+  // one product group per launch (reference op order, no grouping)
+  bool unfused = false;
 
   // launch profiling (RGF_OPT_PROFILE)
   bool profile = false;
