@@ -127,6 +127,16 @@ int rgf_set_workspace(rgf_ctx* ctx, void* ptr, size_t bytes);
  * modified.  n <= 1024. */
 int rgf_inv_z(rgf_ctx* ctx, int64_t n, int64_t n_mats, const void* a, void* out);
 
+/* System-matrix assembly on the device for n_energies energies:
+ *   A_e = z_e I - H - S_B
+ * z: device complex128[n_energies] (E + i eta); h: ONE energy-independent
+ * block-tridiagonal H (diag, upper, lower); sigma_b: NULL or a host array of
+ * n_blocks device pointers to single b_i x b_i retarded boundary blocks
+ * (NULL entries: none).  Mirrors the assembly in the electron solver
+ * (src/quatrex/electron/solver.py:668-709) for a fixed boundary term. */
+int rgf_assemble_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies, const void* z,
+                   const rgf_bt_const* h, const void* const* sigma_b, rgf_bt* a);
+
 /* Selected inversion X = A^{-1} on the tridiagonal pattern, complex128.
  * Replaces RGF.selected_inv (rgf.py:25-113).  obc_retarded: NULL or a host
  * array of n_blocks device pointers (see rgf_obc).  x->upper / x->lower may be
@@ -166,6 +176,17 @@ int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_e
                              const rgf_mixed_opts* opts, rgf_bt* g_retarded, rgf_bt* g_lesser,
                              rgf_bt* g_greater, void* current);
 
+/* GEMM tile configuration for products of b x b blocks over n_energies
+ * energies, chosen by timing every configuration on a representative set of
+ * grouped products (f32 != 0: the FP32-compute products of the mixed path).
+ * The winner is stored in ctx's table for block size b and returned in
+ * best_cfg; ms_per_cfg (optional, RGF_NUM_TILE_CFGS floats) receives the
+ * timings, -1 for configurations that do not apply.  Allocates temporary
+ * device buffers.  No reference counterpart (the autotuner of this build). */
+#define RGF_NUM_TILE_CFGS 4
+int rgf_autotune(rgf_ctx* ctx, int64_t b, int64_t n_energies, int32_t f32, int32_t* best_cfg,
+                 float* ms_per_cfg);
+
 /* Accumulated launch statistics while RGF_OPT_PROFILE is on (synchronises
  * the stream).  Flops are algorithmic: 8*m*n*k real flops per complex
  * product term, 8*n^3 per n x n inversion.  No reference counterpart. */
@@ -178,6 +199,23 @@ typedef struct rgf_profile {
   double inv_flops;
 } rgf_profile;
 int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset);
+
+/* HIP graph of a complete selected-solve sweep for fixed buffers.
+ * rgf_graph_capture_solve records every launch of one sweep over n_energies
+ * energies (precision 0: FP64; 1: FP32-compute products with FP64
+ * inversions and `refine` Newton steps, without the a-posteriori check) into
+ * a graph that owns its workspace and geometry tables; rgf_graph_launch
+ * replays it on the context's stream with whatever data the buffers hold at
+ * that time.  The current is not supported in graphs.  No reference
+ * counterpart (launch-overhead removal for the serving loop). */
+typedef struct rgf_graph rgf_graph;
+int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                            const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
+                            const rgf_bt_const* sigma_greater, const rgf_obc* obc,
+                            int32_t precision, int32_t refine, rgf_bt* g_retarded,
+                            rgf_bt* g_lesser, rgf_bt* g_greater, rgf_graph** out);
+int rgf_graph_launch(rgf_graph* graph);
+int rgf_graph_destroy(rgf_graph* graph);
 
 /* Library version string. */
 const char* rgf_version(void);

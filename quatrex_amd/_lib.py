@@ -25,7 +25,10 @@ RGF_OPT_WORKSPACE_BUDGET = 2
 RGF_OPT_TILE_CFG = 3
 RGF_OPT_PROFILE = 4
 RGF_OPT_UNFUSED = 5
-RGF_OPT_TILE_FOR_BLOCK = 1000  # + block size
+RGF_OPT_TILE_FOR_BLOCK = 1000  # + block size (FP64 products)
+RGF_OPT_TILE_FOR_BLOCK_F32 = 3000  # + block size (FP32-compute products)
+RGF_NUM_TILE_CFGS = 4
+TILE_CFG_NAMES = ("64x64", "32x32", "64x32", "128x64")
 
 # Every function declared in include/rgf.h (checked by the tests).
 EXPORTED_SYMBOLS = (
@@ -38,10 +41,15 @@ EXPORTED_SYMBOLS = (
     "rgf_workspace_size",
     "rgf_set_workspace",
     "rgf_inv_z",
+    "rgf_assemble_z",
     "rgf_selected_inv_z",
     "rgf_selected_solve_z",
     "rgf_selected_solve_mixed",
     "rgf_profile_read",
+    "rgf_autotune",
+    "rgf_graph_capture_solve",
+    "rgf_graph_launch",
+    "rgf_graph_destroy",
     "rgf_version",
 )
 
@@ -107,6 +115,8 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.rgf_workspace_size.argtypes = [P(Geometry_t), c_int64, c_int32, c_int32, c_int32, P(c_size_t)]
     lib.rgf_set_workspace.argtypes = [c_void_p, c_void_p, c_size_t]
     lib.rgf_inv_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p]
+    lib.rgf_assemble_z.argtypes = [c_void_p, P(Geometry_t), c_int64, c_void_p, P(BtConst_t),
+                                   P(c_void_p), P(Bt_t)]
     lib.rgf_selected_inv_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t), P(c_void_p),
                                        P(Bt_t)]
     lib.rgf_selected_solve_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
@@ -116,6 +126,12 @@ def _declare(lib: ctypes.CDLL) -> None:
                                              P(BtConst_t), P(BtConst_t), P(Obc_t),
                                              P(MixedOpts_t), P(Bt_t), P(Bt_t), P(Bt_t), c_void_p]
     lib.rgf_profile_read.argtypes = [c_void_p, P(Profile_t), c_int32]
+    lib.rgf_autotune.argtypes = [c_void_p, c_int64, c_int64, c_int32, P(c_int32), P(ctypes.c_float)]
+    lib.rgf_graph_capture_solve.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
+                                            P(BtConst_t), P(BtConst_t), P(Obc_t), c_int32, c_int32,
+                                            P(Bt_t), P(Bt_t), P(Bt_t), P(c_void_p)]
+    lib.rgf_graph_launch.argtypes = [c_void_p]
+    lib.rgf_graph_destroy.argtypes = [c_void_p]
     lib.rgf_version.argtypes = []
     lib.rgf_version.restype = c_char_p
     for name in EXPORTED_SYMBOLS:

@@ -6,22 +6,43 @@ namespace qx {
 namespace {
 
 // X <- (X - X^H)/2 in place, for nblk diagonal blocks (offset offs[i], size
-// sizes[i]) of each energy.  Each thread owns one (r, c) pair with r <= c.
+// sizes[i]) of each energy.  A workgroup owns a pair of mirrored 32 x 32 tiles
+// (ti <= tj): both are read row-coalesced into LDS, combined, and written
+// back row-coalesced.
 __global__ void __launch_bounds__(256)
 project_kernel(double2* base, int64_t sE, const int64_t* offs, const int* sizes) {
+  __shared__ double2 ta[32][33];
+  __shared__ double2 tb[32][33];
   const int blk = blockIdx.y, e = blockIdx.z;
   const int b = sizes[blk];
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= b * b) return;
-  const int r = idx / b, c = idx % b;
-  if (r > c) return;
+  const int T = (b + 31) / 32;
+  int p = blockIdx.x, ti = 0;
+  while (ti < T && p >= T - ti) { p -= T - ti; ++ti; }
+  if (ti >= T) return;
+  const int tj = ti + p;
   double2* X = base + (int64_t)e * sE + offs[blk];
-  const double2 u = X[(int64_t)r * b + c];
-  const double2 l = X[(int64_t)c * b + r];
-  // (u - conj(l))/2 at (r,c); (l - conj(u))/2 at (c,r)
-  const double2 nu = cmk(0.5 * (u.x - l.x), 0.5 * (u.y + l.y));
-  X[(int64_t)r * b + c] = nu;
-  if (r != c) X[(int64_t)c * b + r] = cmk(-nu.x, nu.y);
+  const int x = threadIdx.x & 31, y0 = threadIdx.x >> 5;
+  for (int y = y0; y < 32; y += 8) {
+    const int r = ti * 32 + y, c = tj * 32 + x;
+    ta[y][x] = (r < b && c < b) ? X[(int64_t)r * b + c] : cmk(0.0, 0.0);
+    const int r2 = tj * 32 + y, c2 = ti * 32 + x;
+    tb[y][x] = (r2 < b && c2 < b) ? X[(int64_t)r2 * b + c2] : cmk(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int y = y0; y < 32; y += 8) {
+    const int r = ti * 32 + y, c = tj * 32 + x;
+    if (r < b && c < b) {
+      const double2 u = ta[y][x], l = tb[x][y];
+      X[(int64_t)r * b + c] = cmk(0.5 * (u.x - l.x), 0.5 * (u.y + l.y));
+    }
+    if (ti != tj) {
+      const int r2 = tj * 32 + y, c2 = ti * 32 + x;
+      if (r2 < b && c2 < b) {
+        const double2 u = tb[y][x], l = ta[x][y];
+        X[(int64_t)r2 * b + c2] = cmk(0.5 * (u.x - l.x), 0.5 * (u.y + l.y));
+      }
+    }
+  }
 }
 
 // out[e*sOut + slot] = sign * (tr(X1 Y1) - tr(X2 Y2)),  tr(XY) = sum_rc X[r][c] Y[c][r].
@@ -82,6 +103,33 @@ __global__ void identity_kernel(double2* X, int n) {
   if (idx < n * n) X[idx] = cmk((idx / n) == (idx % n) ? 1.0 : 0.0, 0.0);
 }
 
+// A = -H for every energy (one copy of H, energy stride 0 on input).
+__global__ void neg_copy_kernel(const double2* __restrict__ h, double2* __restrict__ a, int64_t n,
+                                int64_t sA) {
+  const int e = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = h[i];
+    a[(int64_t)e * sA + i] = cmk(-v.x, -v.y);
+  }
+}
+
+// diag blocks: A_ii += z_e I - S_i  (S_i optional b_i x b_i, shared by energies)
+__global__ void shift_diag_kernel(double2* a, int64_t sA, const double2* z, const int64_t* offs,
+                                  const int* sizes, const double2* const* sig) {
+  const int blk = blockIdx.y, e = blockIdx.z;
+  const int b = sizes[blk];
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= b * b) return;
+  const int r = idx / b, c = idx % b;
+  double2* A = a + (int64_t)e * sA + offs[blk];
+  double2 v = A[idx];
+  if (r == c) v = cadd(v, z[e]);
+  const double2* S = sig[blk];
+  if (S) v = csub(v, S[idx]);
+  A[idx] = v;
+}
+
 __global__ void z2c_kernel(const double2* src, int64_t sSrc, float2* dst, int64_t sDst,
                            int64_t count) {
   const int e = blockIdx.y;
@@ -107,7 +155,8 @@ __global__ void c2z_kernel(const float2* src, int64_t sSrc, double2* dst, int64_
 hipError_t launch_project(double2* base, int64_t sE, const int64_t* offs_dev, const int* sizes_dev,
                           int nblk, int bmax, int nE, hipStream_t stream) {
   if (nblk == 0 || nE == 0) return hipSuccess;
-  dim3 grid((bmax * bmax + 255) / 256, nblk, nE);
+  const int T = (bmax + 31) / 32;
+  dim3 grid(T * (T + 1) / 2, nblk, nE);
   hipLaunchKernelGGL(project_kernel, grid, dim3(256), 0, stream, base, sE, offs_dev, sizes_dev);
   return hipGetLastError();
 }
@@ -128,6 +177,25 @@ hipError_t launch_norms(const NormJob* jobs_dev, int njobs, int nE, double* out,
 
 hipError_t launch_identity(double2* X, int n, hipStream_t stream) {
   hipLaunchKernelGGL(identity_kernel, dim3((n * n + 255) / 256), dim3(256), 0, stream, X, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble(const double2* h_diag, const double2* h_upper, const double2* h_lower,
+                           int64_t Dsz, int64_t Usz, int nE, const double2* z, const int64_t* offs,
+                           const int* sizes, const double2* const* sig, int nblk, int bmax,
+                           double2* a_diag, double2* a_upper, double2* a_lower, hipStream_t stream) {
+  if (nE == 0) return hipSuccess;
+  auto grid = [](int64_t n, int ne) {
+    int64_t gx = (n + 255) / 256;
+    return dim3((unsigned)(gx > 2048 ? 2048 : gx), (unsigned)ne);
+  };
+  hipLaunchKernelGGL(neg_copy_kernel, grid(Dsz, nE), dim3(256), 0, stream, h_diag, a_diag, Dsz, Dsz);
+  if (Usz > 0) {
+    hipLaunchKernelGGL(neg_copy_kernel, grid(Usz, nE), dim3(256), 0, stream, h_upper, a_upper, Usz, Usz);
+    hipLaunchKernelGGL(neg_copy_kernel, grid(Usz, nE), dim3(256), 0, stream, h_lower, a_lower, Usz, Usz);
+  }
+  dim3 g2((bmax * bmax + 255) / 256, nblk, nE);
+  hipLaunchKernelGGL(shift_diag_kernel, g2, dim3(256), 0, stream, a_diag, Dsz, z, offs, sizes, sig);
   return hipGetLastError();
 }
 

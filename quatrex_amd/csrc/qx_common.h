@@ -172,6 +172,12 @@ hipError_t launch_norms(const NormJob* jobs_dev, int njobs, int nE, double* out,
 // n x n identity (one matrix; use energy stride 0 to share it).
 hipError_t launch_identity(double2* X, int n, hipStream_t stream);
 
+// System-matrix assembly A = z_e I - H - S (per energy e; H and S shared).
+hipError_t launch_assemble(const double2* h_diag, const double2* h_upper, const double2* h_lower,
+                           int64_t Dsz, int64_t Usz, int nE, const double2* z, const int64_t* offs,
+                           const int* sizes, const double2* const* sig, int nblk, int bmax,
+                           double2* a_diag, double2* a_upper, double2* a_lower, hipStream_t stream);
+
 // Precision conversion helpers for the mixed path.
 hipError_t launch_z2c(const double2* src, int64_t sSrc, float2* dst, int64_t sDst, int64_t count,
                       int nE, hipStream_t stream);

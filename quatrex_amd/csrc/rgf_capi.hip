@@ -103,7 +103,8 @@ int Ctx::ensure_meta(size_t bytes) {
 
 int Ctx::tile_for(int b) const {
   if (tile_override >= 0) return tile_override;
-  if (b >= 0 && b < (int)tile_table.size() && tile_table[b] >= 0) return tile_table[b];
+  const std::vector<int>& t = prec32 ? tile_table32 : tile_table;
+  if (b >= 0 && b < (int)t.size() && t[b] >= 0) return t[b];
   return b < 64 ? kTile32x32 : kTile64x64;
 }
 
@@ -871,6 +872,21 @@ int64_t Ctx::chunk_for(const Geo& g, int64_t nE, bool solve, bool current, bool 
 
 struct rgf_ctx : public qx::Ctx {};
 
+struct rgf_graph {
+  rgf_ctx* ctx = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void* ws = nullptr;
+  void* tables = nullptr;
+  ~rgf_graph() {
+    if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ws) (void)hipFree(ws);
+    if (tables) (void)hipFree(tables);
+  }
+};
+
 #define GUARD_BEGIN try {
 #define GUARD_END                                              \
   }                                                            \
@@ -944,12 +960,16 @@ int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value) {
       ctx->tile_override = (int)value;
       return RGF_OK;
     default:
-      // 1000 + b: tile config for block size b (autotuner table)
-      if (option >= 1000 && option < 1000 + 1025) {
-        const int b = option - 1000;
+      // 1000 + b (FP64) / 3000 + b (FP32 compute): tile config for block size b
+      if ((option >= 1000 && option < 1000 + 1025) || (option >= 3000 && option < 3000 + 1025)) {
+        const bool f32 = option >= 3000;
+        const int b = option - (f32 ? 3000 : 1000);
         if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
-        if ((int)ctx->tile_table.size() <= b) ctx->tile_table.resize(b + 1, -1);
-        ctx->tile_table[b] = (int)value;
+        if (f32 && value != qx::kTile32x32 && value != qx::kTile64x64 && value >= 0)
+          return ctx->fail(RGF_EINVAL, "FP32-compute products support tile configs 0 and 1");
+        std::vector<int>& t = f32 ? ctx->tile_table32 : ctx->tile_table;
+        if ((int)t.size() <= b) t.resize(b + 1, -1);
+        t[b] = (int)value;
         return RGF_OK;
       }
       return ctx->fail(RGF_EINVAL, "unknown option");
@@ -1038,6 +1058,40 @@ int rgf_inv_z(rgf_ctx* ctx, int64_t n, int64_t n_mats, const void* a, void* out)
     if (rc) return rc;
   }
   return RGF_OK;
+  GUARD_END
+}
+
+int rgf_assemble_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies, const void* z,
+                   const rgf_bt_const* h, const void* const* sigma_b, rgf_bt* a) {
+  if (!ctx) return RGF_EINVAL;
+  GUARD_BEGIN
+  qx::Geo g;
+  int rc = g.init(geom, &ctx->last_error);
+  if (rc) return rc;
+  if (n_energies < 0) return ctx->fail(RGF_EINVAL, "n_energies < 0");
+  if (n_energies == 0) return RGF_OK;
+  if (!z || !h || !h->diag || !a || !a->diag || (g.N > 1 && (!h->upper || !h->lower || !a->upper || !a->lower)))
+    return ctx->fail(RGF_EINVAL, "assemble: z, H and A buffers required");
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  rc = ctx->upload_geometry(g);
+  if (rc) return rc;
+  // per-block boundary pointers go to the meta area after the geometry tables
+  std::vector<const double2*> sig(g.N, nullptr);
+  if (sigma_b)
+    for (int i = 0; i < g.N; ++i) sig[i] = static_cast<const double2*>(sigma_b[i]);
+  const size_t bytes = g.N * sizeof(double2*);
+  if (ctx->meta_used + bytes > ctx->meta_bytes) return ctx->fail(RGF_EHIP, "meta area too small");
+  const double2** dsig = reinterpret_cast<const double2**>(static_cast<char*>(ctx->meta) + ctx->meta_used);
+  rc = ctx->hip(hipMemcpyAsync(dsig, sig.data(), bytes, hipMemcpyHostToDevice, ctx->stream), "sigma_b table");
+  if (rc) return rc;
+  rc = ctx->hip(hipStreamSynchronize(ctx->stream), "sigma_b table sync");
+  if (rc) return rc;
+  return ctx->hip(launch_assemble((const double2*)h->diag, (const double2*)h->upper,
+                                  (const double2*)h->lower, g.Dsz, g.Usz, (int)n_energies,
+                                  (const double2*)z, ctx->offs_dev, ctx->sizes_dev, dsig, g.N,
+                                  (int)g.bmax, (double2*)a->diag, (double2*)a->upper,
+                                  (double2*)a->lower, ctx->stream),
+                  "assemble");
   GUARD_END
 }
 
@@ -1130,6 +1184,173 @@ int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energ
   }
   return RGF_OK;
   GUARD_END
+}
+
+int rgf_autotune(rgf_ctx* ctx, int64_t b, int64_t n_energies, int32_t f32, int32_t* best_cfg,
+                 float* ms_per_cfg) {
+  if (!ctx || !best_cfg) return RGF_EINVAL;
+  GUARD_BEGIN
+  if (b < 1 || b > 1024 || n_energies < 1) return ctx->fail(RGF_EINVAL, "autotune: bad sizes");
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  const size_t el = (size_t)b * b * n_energies;
+  double2 *A = nullptr, *B = nullptr, *D = nullptr;
+  int rc = ctx->hip(hipMalloc(&A, el * sizeof(double2)), "autotune alloc");
+  if (!rc) rc = ctx->hip(hipMalloc(&B, el * sizeof(double2)), "autotune alloc");
+  if (!rc) rc = ctx->hip(hipMalloc(&D, el * sizeof(double2)), "autotune alloc");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc) rc = ctx->hip(hipEventCreate(&e0), "event");
+  if (!rc) rc = ctx->hip(hipEventCreate(&e1), "event");
+  if (!rc) rc = ctx->hip(hipMemsetAsync(A, 0, el * sizeof(double2), ctx->stream), "memset");
+  if (!rc) rc = ctx->hip(hipMemsetAsync(B, 0, el * sizeof(double2), ctx->stream), "memset");
+  float best = 1e30f;
+  int best_c = -1;
+  const int64_t s = b * b;
+  // a representative step: three products (N.N, N.C with an addend, two-term)
+  std::vector<ZProblem> probs = {
+      PB((int)b, (int)b).term(Mat{A, s, (int)b}, kOpN, Mat{B, s, (int)b}, kOpN, (int)b).out(Mat{D, s, (int)b}).done(),
+      PB((int)b, (int)b).term(Mat{A, s, (int)b}, kOpN, Mat{B, s, (int)b}, kOpC, (int)b)
+          .add(Mat{A, s, (int)b}, cmk(1.0, 0.0)).out(Mat{D, s, (int)b}).done(),
+      PB((int)b, (int)b).term(Mat{B, s, (int)b}, kOpN, Mat{A, s, (int)b}, kOpN, (int)b)
+          .term(Mat{A, s, (int)b}, kOpC, Mat{B, s, (int)b}, kOpN, (int)b, cmk(-1.0, 0.0))
+          .out(Mat{D, s, (int)b}).done()};
+  const bool saved32 = ctx->prec32;
+  const int saved_over = ctx->tile_override;
+  const bool saved_prof = ctx->profile;
+  ctx->prec32 = f32 != 0;
+  ctx->profile = false;
+  for (int cfg = 0; cfg < qx::kNumTileCfgs && !rc; ++cfg) {
+    const bool ok = f32 ? (cfg == qx::kTile32x32 || cfg == qx::kTile64x64) : true;
+    if (!ok) {
+      if (ms_per_cfg) ms_per_cfg[cfg] = -1.f;
+      continue;
+    }
+    ctx->tile_override = cfg;
+    rc = ctx->gemm(probs, (int)n_energies, (int)b);  // warm-up
+    if (!rc) rc = ctx->hip(hipEventRecord(e0, ctx->stream), "event");
+    for (int r = 0; r < 3 && !rc; ++r) rc = ctx->gemm(probs, (int)n_energies, (int)b);
+    if (!rc) rc = ctx->hip(hipEventRecord(e1, ctx->stream), "event");
+    if (!rc) rc = ctx->hip(hipEventSynchronize(e1), "event sync");
+    float ms = 0.f;
+    if (!rc) rc = ctx->hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    ms /= 3.f;
+    if (ms_per_cfg) ms_per_cfg[cfg] = ms;
+    if (!rc && ms < best) { best = ms; best_c = cfg; }
+  }
+  ctx->prec32 = saved32;
+  ctx->tile_override = saved_over;
+  ctx->profile = saved_prof;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (A) (void)hipFree(A);
+  if (B) (void)hipFree(B);
+  if (D) (void)hipFree(D);
+  if (rc) return rc;
+  *best_cfg = best_c;
+  std::vector<int>& t = f32 ? ctx->tile_table32 : ctx->tile_table;
+  if ((int)t.size() <= b) t.resize(b + 1, -1);
+  t[b] = best_c;
+  return RGF_OK;
+  GUARD_END
+}
+
+int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                            const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
+                            const rgf_bt_const* sigma_greater, const rgf_obc* obc,
+                            int32_t precision, int32_t refine, rgf_bt* g_retarded,
+                            rgf_bt* g_lesser, rgf_bt* g_greater, rgf_graph** out) {
+  if (!ctx || !out) return RGF_EINVAL;
+  *out = nullptr;
+  GUARD_BEGIN
+  qx::Geo g;
+  int rc = g.init(geom, &ctx->last_error);
+  if (rc) return rc;
+  if (n_energies < 1 || n_energies > 65535) return ctx->fail(RGF_EINVAL, "graph: 1 <= n_energies <= 65535");
+  if (precision != 0 && precision != 1) return ctx->fail(RGF_EINVAL, "graph: precision 0 or 1");
+  const bool multi = g.N > 1;
+  if (!a || !a->diag || (multi && (!a->upper || !a->lower)) || !sigma_lesser || !sigma_greater ||
+      !sigma_lesser->diag || !sigma_greater->diag ||
+      (multi && (!sigma_lesser->upper || !sigma_greater->upper)) || !g_lesser || !g_greater ||
+      !g_lesser->diag || !g_greater->diag || (multi && (!g_lesser->upper || !g_greater->upper)))
+    return ctx->fail(RGF_EINVAL, "graph: missing buffers");
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  rgf_graph* gr = new rgf_graph();
+  gr->ctx = ctx;
+  const bool have_gr = g_retarded && g_retarded->diag;
+  const bool mixed = precision == 1;
+  const size_t wbytes = work_bytes(g, n_energies, true, false, have_gr, mixed, true);
+  const size_t tbytes = g.N * (sizeof(int64_t) + sizeof(int));
+  rc = ctx->hip(hipMalloc(&gr->ws, wbytes), "graph workspace");
+  if (!rc) rc = ctx->hip(hipMalloc(&gr->tables, tbytes), "graph tables");
+  if (rc) { delete gr; return rc; }
+  {
+    std::vector<char> host(tbytes);
+    std::memcpy(host.data(), g.od.data(), g.N * sizeof(int64_t));
+    std::vector<int> sz(g.N);
+    for (int i = 0; i < g.N; ++i) sz[i] = (int)g.bs[i];
+    std::memcpy(host.data() + g.N * sizeof(int64_t), sz.data(), g.N * sizeof(int));
+    rc = ctx->hip(hipMemcpy(gr->tables, host.data(), tbytes, hipMemcpyHostToDevice), "graph tables copy");
+    if (rc) { delete gr; return rc; }
+  }
+  Work w;
+  carve(g, n_energies, true, false, have_gr, static_cast<char*>(gr->ws), &w, mixed, true);
+  if (mixed) {
+    rc = ctx->hip(launch_identity(w.ident, (int)g.bmax, ctx->stream), "identity");
+    if (!rc) rc = ctx->hip(hipStreamSynchronize(ctx->stream), "identity sync");
+    if (rc) { delete gr; return rc; }
+  }
+  SolveIO io;
+  io.A = view(g, a->diag, a->upper, a->lower, 0);
+  io.SL = view(g, sigma_lesser->diag, sigma_lesser->upper, nullptr, 0);
+  io.SG = view(g, sigma_greater->diag, sigma_greater->upper, nullptr, 0);
+  io.obc = obc;
+  io.GR = have_gr ? view(g, g_retarded->diag, g_retarded->upper, g_retarded->lower, 0)
+                  : BTV{&g, nullptr, nullptr, nullptr, 0, -1, -1};
+  io.GL = view(g, g_lesser->diag, g_lesser->upper, g_lesser->lower, 0);
+  io.GG = view(g, g_greater->diag, g_greater->upper, g_greater->lower, 0);
+  io.current = nullptr;
+  // record with the graph's own tables and flags; restore the context afterwards
+  int64_t* saved_offs = ctx->offs_dev;
+  int* saved_sizes = ctx->sizes_dev;
+  const bool saved_prof = ctx->profile, saved32 = ctx->prec32;
+  const int saved_ref = ctx->refine;
+  ctx->offs_dev = static_cast<int64_t*>(gr->tables);
+  ctx->sizes_dev = reinterpret_cast<int*>(static_cast<char*>(gr->tables) + g.N * sizeof(int64_t));
+  ctx->profile = false;
+  ctx->prec32 = mixed;
+  ctx->refine = mixed ? refine : 0;
+  hipGraph_t graph = nullptr;
+  rc = ctx->hip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  int rc_sweep = RGF_OK;
+  if (!rc) {
+    rc_sweep = ctx->reset_info();
+    if (!rc_sweep) rc_sweep = sweep_chunk_solve(ctx, g, 0, (int)n_energies, io, w);
+    rc = ctx->hip(hipStreamEndCapture(ctx->stream, &graph), "end capture");
+  }
+  ctx->offs_dev = saved_offs;
+  ctx->sizes_dev = saved_sizes;
+  ctx->profile = saved_prof;
+  ctx->prec32 = saved32;
+  ctx->refine = saved_ref;
+  if (rc_sweep) rc = rc_sweep;
+  if (!rc) rc = ctx->hip(hipGraphInstantiate(&gr->exec, graph, nullptr, nullptr, 0), "instantiate");
+  gr->graph = graph;
+  if (rc) { delete gr; return rc; }
+  *out = gr;
+  return RGF_OK;
+  GUARD_END
+}
+
+int rgf_graph_launch(rgf_graph* graph) {
+  if (!graph || !graph->exec) return RGF_EINVAL;
+  rgf_ctx* ctx = graph->ctx;
+  return ctx->hip(hipGraphLaunch(graph->exec, ctx->stream), "graph launch");
+}
+
+int rgf_graph_destroy(rgf_graph* graph) {
+  if (!graph) return RGF_OK;
+  delete graph;
+  return RGF_OK;
 }
 
 int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset) {

@@ -44,7 +44,8 @@ struct Ctx {
   size_t ws_budget = (size_t)4 << 30;
   int64_t max_chunk = 0;
   int tile_override = -1;
-  std::vector<int> tile_table;
+  std::vector<int> tile_table;    // FP64 products, by block size
+  std::vector<int> tile_table32;  // FP32-compute products, by block size
   int* info_dev = nullptr;
   void* meta = nullptr;
   size_t meta_bytes = 0;

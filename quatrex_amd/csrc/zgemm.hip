@@ -173,7 +173,9 @@ zgemm_grouped_kernel(const ZGroup g) {
     }
 
   for (int ti = 0; ti < P.nterms; ++ti) {
-    const ZTerm& T = P.t[ti];
+    // by value: the term's fields are loaded once into scalar registers
+    // instead of being re-read from the kernel-argument segment in the loop
+    const ZTerm T = P.t[ti];
     const int nk = (T.k + TK - 1) / TK;
     double2 ra[S::kAPer], rb[S::kBPer];
     load_a<TM, TN>(T, e, m, m0, 0, tid, ra);

@@ -297,6 +297,84 @@ def inv(a, out=None, ctx: Context | None = None):
     return out
 
 
+def assemble(geometry: Geometry, z, h, *, a, sigma_b=None, ctx: Context | None = None) -> None:
+    """``A_e = z_e I - H - S_B`` on the device (asynchronous).
+
+    ``z``: ``(E,)`` complex128 device tensor; ``h``: ``(diag, upper, lower)`` of
+    ONE energy-independent H (1-D packed tensors); ``sigma_b``: ``None`` or a
+    per-block list of ``(b_i, b_i)`` device tensors; ``a``: packed outputs.
+    """
+    ctx = ctx or default_context()
+    geo = geometry
+    E = int(z.shape[0])
+    _check(z, (E,), "z")
+    H = _lib.BtConst_t(*(_check(t, (n,), f"h.{p}") if t is not None else None
+                         for t, n, p in zip(h, (geo.diag_size, geo.offdiag_size, geo.offdiag_size),
+                                            ("diag", "upper", "lower"))))
+    Aout = _bt(a, geo, E, "a", const=False)
+    keep = None
+    sb = None
+    if sigma_b is not None:
+        for i, t in enumerate(sigma_b):
+            if t is not None:
+                b = geo.block_sizes[i]
+                _check(t, (b, b), f"sigma_b[{i}]")
+        sb, keep = _lib.ptr_array([t.data_ptr() if t is not None else None for t in sigma_b])
+    gstruct, gkeep = _lib.geometry_struct(geo.block_sizes)
+    rc = _lib.load().rgf_assemble_z(ctx.handle, ctypes.byref(gstruct), E, ctypes.c_void_p(z.data_ptr()),
+                                    ctypes.byref(H), sb, ctypes.byref(Aout))
+    del keep, gkeep
+    _lib.check(rc, ctx.handle)
+
+
+class SolveGraph:
+    """A captured selected-solve sweep for fixed device buffers (hipGraph).
+
+    ``launch()`` replays the sweep on the context stream with the data the
+    buffers hold at that time (e.g. after re-assembling ``A`` in place).
+    """
+
+    def __init__(self, geometry: Geometry, a, sigma_lesser, sigma_greater, *, g_lesser, g_greater,
+                 g_retarded=None, obc=None, ctx: Context | None = None, precision: str = "fp64",
+                 refine: int = 1):
+        self.ctx = ctx or default_context()
+        geo = geometry
+        E = int(a[0].shape[0])
+        A = _bt(a, geo, E, "a")
+        SL = _bt(sigma_lesser, geo, E, "sigma_lesser", (True, True, False))
+        SG = _bt(sigma_greater, geo, E, "sigma_greater", (True, True, False))
+        GL = _bt(g_lesser, geo, E, "g_lesser", (True, True, False), const=False)
+        GG = _bt(g_greater, geo, E, "g_greater", (True, True, False), const=False)
+        GR = (_bt(g_retarded, geo, E, "g_retarded", (True, False, False), const=False)
+              if g_retarded is not None else None)
+        obc_s, self._keep = _obc_struct(_obc_lists(obc, geo, E))
+        gstruct, gkeep = _lib.geometry_struct(geo.block_sizes)
+        h = ctypes.c_void_p()
+        byref = ctypes.byref
+        rc = _lib.load().rgf_graph_capture_solve(
+            self.ctx.handle, byref(gstruct), E, byref(A), byref(SL), byref(SG),
+            byref(obc_s) if obc_s else None, 1 if precision == "mixed" else 0, int(refine),
+            byref(GR) if GR else None, byref(GL), byref(GG), byref(h))
+        _lib.check(rc, self.ctx.handle)
+        self._h = h
+        # keep the buffers alive as long as the graph references them
+        self._buffers = (a, sigma_lesser, sigma_greater, g_lesser, g_greater, g_retarded, obc)
+
+    def launch(self) -> None:
+        _lib.check(_lib.load().rgf_graph_launch(self._h), self.ctx.handle)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _lib.load().rgf_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def workspace_size(geometry: Geometry, n_energies: int, solve: bool = True, current: bool = False,
                    retarded: bool = True) -> int:
     """Device workspace bytes for one sweep over ``n_energies`` energies."""

@@ -1,0 +1,101 @@
+"""hipGraph-captured sweeps (replayed with new data, checked against eager
+solves after every replay) and the block-size GEMM autotuner."""
+
+import json
+
+import numpy as np
+import pytest
+
+from quatrex_amd import autotune
+from quatrex_amd import solve as S
+from quatrex_amd import workloads as W
+from quatrex_amd.layout import Geometry
+
+pytestmark = pytest.mark.gpu
+
+
+def _resident(g, seed, device):
+    import torch
+
+    m = W.make_device(g, seed)
+    t = lambda x: torch.as_tensor(x, device=device)  # noqa: E731
+    sb = [None] * g.n_blocks
+    for i in W.boundary_blocks(g):
+        sb[i] = t(W.BOUNDARY_SIGMA * np.eye(g.block_sizes[i], dtype=np.complex128))
+    return m, (t(m.h_diag), t(m.h_upper), t(m.h_lower)), sb
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_match_eager(device, precision):
+    import torch
+
+    g = Geometry((48, 40, 48, 32, 48))
+    E = 3
+    m, h, sb = _resident(g, 77, device)
+    z = torch.zeros(E, dtype=torch.complex128, device=device)
+    A = S.packed_empty(g, E, device)
+    SL = (torch.as_tensor(m.sl_diag, device=device).expand(E, -1).contiguous(),
+          torch.zeros((E, g.offdiag_size), dtype=torch.complex128, device=device))
+    SG = (torch.as_tensor(m.sg_diag, device=device).expand(E, -1).contiguous(),
+          torch.zeros((E, g.offdiag_size), dtype=torch.complex128, device=device))
+    outs = [S.packed_empty(g, E, device), S.packed_empty(g, E, device, ("d", "u")),
+            S.packed_empty(g, E, device, ("d", "u"))]
+    ref = [S.packed_empty(g, E, device), S.packed_empty(g, E, device, ("d", "u")),
+           S.packed_empty(g, E, device, ("d", "u"))]
+    ctx = S.Context(device)
+    graph = S.SolveGraph(g, A, SL, SG, g_retarded=outs[0], g_lesser=outs[1], g_greater=outs[2],
+                         ctx=ctx, precision=precision, refine=1)
+    rng = np.random.default_rng(3)
+    for rep in range(3):
+        en = rng.uniform(-1, 1, E)
+        z.copy_(torch.as_tensor(en + 1e-3j, dtype=torch.complex128))
+        S.assemble(g, z, h, a=A, sigma_b=sb, ctx=ctx)
+        graph.launch()
+        ctx.status()
+        # eager reference on the same inputs (other buffers), with a sync in between
+        kw = {} if precision == "fp64" else dict(precision="mixed",
+                                                 mixed=S.MixedOptions(refine=1, fallback_tol=0.0))
+        S.selected_solve(g, A, SL, SG, g_retarded=ref[0], g_lesser=ref[1], g_greater=ref[2], ctx=ctx, **kw)
+        ctx.status()
+        for X, R in zip(outs, ref):
+            for x, r in zip(X, R):
+                if x is not None:
+                    assert torch.equal(x, r), rep
+        # eager work between replays
+        (A[0] * 2).sum().item()
+    graph.close()
+    ctx.close()
+
+
+def test_autotune_picks_valid_config_and_caches(device, tmp_path):
+    ctx = S.Context(device)
+    path = tmp_path / "tiles.json"
+    e1 = autotune.tune(ctx, 64, n_energies=32, path=path)
+    assert e1["best"] in (0, 1, 2, 3) and e1["best_name"] in e1["ms"]
+    assert all(v > 0 for v in e1["ms"].values())
+    e2 = autotune.tune(ctx, 64, n_energies=32, f32=True, path=path)
+    assert e2["best"] in (0, 1)
+    db = json.loads(path.read_text())
+    assert len(next(iter(db.values()))) == 2
+    # cached: no re-timing
+    assert autotune.tune(ctx, 64, path=path) == e1
+    # a tuned context still solves correctly
+    g = Geometry.uniform(4, 64)
+    m, h, sb = _resident(g, 5, device)
+    import torch
+
+    z = torch.tensor([0.2 + 1e-3j], dtype=torch.complex128, device=device)
+    A = S.packed_empty(g, 1, device)
+    S.assemble(g, z, h, a=A, sigma_b=sb, ctx=ctx)
+    SL = (torch.as_tensor(m.sl_diag, device=device)[None].contiguous(),
+          torch.zeros((1, g.offdiag_size), dtype=torch.complex128, device=device))
+    GR, GL, GG = S.packed_empty(g, 1, device), S.packed_empty(g, 1, device, ("d", "u")), S.packed_empty(g, 1, device, ("d", "u"))
+    S.selected_solve(g, A, SL, SL, g_retarded=GR, g_lesser=GL, g_greater=GG, ctx=ctx)
+    ctx.status()
+    Ah = tuple(x.cpu().numpy() for x in A)
+    dense = np.linalg.inv(g.to_dense(*Ah))[0]
+    blk = g.diag(GR[0].cpu().numpy(), 2)[0]
+    bo = g.block_offsets
+    ref = dense[bo[2]:bo[3], bo[2]:bo[3]]
+    assert np.linalg.norm(blk - ref) / np.linalg.norm(ref) < 1e-10
+    ctx.close()

@@ -1,0 +1,91 @@
+"""Serving loop on the GPU: concurrent batches of several geometries, and each
+request's result equal to the same request solved alone."""
+
+import numpy as np
+import pytest
+
+from quatrex_amd import solve as S
+from quatrex_amd.layout import Geometry
+from quatrex_amd.serving import AdmissionError, Request, Server, latency_stats
+
+pytestmark = pytest.mark.gpu
+
+GEOS = [Geometry.uniform(6, 32), Geometry.uniform(4, 48), Geometry.uniform(5, 64)]
+SEEDS = [51, 52, 53]
+
+
+def _alone(server, gid, req, precision):
+    """Solve one request in a fresh single-energy batch on the default stream."""
+    import torch
+
+    res = server.resident[gid]
+    g = res.geometry
+    dev = server.device
+    z = torch.tensor([req.energy + 1j * req.eta], dtype=torch.complex128, device=dev)
+    A = S.packed_empty(g, 1, dev)
+    S.assemble(g, z, res.h, a=A, sigma_b=res.sigma_b)
+    GR, GL, GG = S.packed_empty(g, 1, dev), S.packed_empty(g, 1, dev, ("d", "u")), S.packed_empty(g, 1, dev, ("d", "u"))
+    SL = (res.sl_diag[None].contiguous(), res.sl_upper[None].contiguous())
+    SG = (res.sg_diag[None].contiguous(), res.sg_upper[None].contiguous())
+    if precision == "fp64":
+        S.selected_solve(g, A, SL, SG, g_lesser=GL, g_greater=GG, g_retarded=GR)
+    else:
+        S.selected_solve(g, A, SL, SG, g_lesser=GL, g_greater=GG, g_retarded=GR, precision="mixed",
+                         mixed=S.MixedOptions(refine=1, fallback_tol=1e-8))
+    S.default_context().status()
+    return GR, GL, GG
+
+
+@pytest.mark.parametrize("precision", ["fp64", "adaptive"])
+def test_results_same_alone_and_in_the_mix(device, precision):
+    import torch
+
+    rng = np.random.default_rng(9)
+    n = 36
+    reqs = [Request(0.0, k, int(rng.integers(0, 3)), float(rng.uniform(-1, 1))) for k in range(n)]
+    keep = {0, 5, 11, 17, 23, 35}
+    srv = Server(GEOS, SEEDS, budget_bytes=8 << 30, capacity=4, slots_per_geometry=2,
+                 precision=precision, keep_outputs=keep, device=device)
+    try:
+        results = srv.run(reqs, realtime=False)
+        assert sorted(r.rid for r in results) == list(range(n))
+        st = latency_stats(results)
+        assert st["batches"] >= n // 4
+        for rid in keep:
+            req = reqs[rid]
+            got = srv.kept[rid]
+            ref = _alone(srv, req.geometry, req, "fp64" if precision == "fp64" else "mixed")
+            for X, R in zip(got, ref):
+                for part, rpart in zip(X, R):
+                    if part is None or rpart is None:
+                        continue
+                    r = rpart[0]
+                    if precision == "fp64":
+                        assert torch.equal(part, r)
+                    else:
+                        err = float(torch.linalg.norm(part - r) / torch.linalg.norm(r))
+                        assert err <= 1e-8, (rid, err)
+    finally:
+        srv.close()
+
+
+def test_admission_by_footprint(device):
+    with pytest.raises(AdmissionError):
+        Server([Geometry.uniform(32, 256)], [1], budget_bytes=64 << 20, capacity=4,
+               slots_per_geometry=1, device=device)
+
+
+def test_realtime_poisson_stream(device):
+    from quatrex_amd.serving import poisson_requests
+
+    reqs = poisson_requests(60, 400.0, 3, seed=7)
+    srv = Server(GEOS, SEEDS, budget_bytes=8 << 30, capacity=4, slots_per_geometry=2,
+                 precision="adaptive", device=device)
+    try:
+        results = srv.run(reqs, realtime=True)
+        st = latency_stats(results)
+        assert st["requests"] == 60
+        assert all(r.done >= r.arrival for r in results)
+        assert st["p99_ms"] >= st["p50_ms"] > 0
+    finally:
+        srv.close()
