@@ -53,8 +53,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
+    ap.add_argument("--torch-sample", type=int, default=100,
+                    help="energies of the torch (rocBLAS/rocSOLVER) reference-algorithm leg (0 = skip)")
     ap.add_argument("--plain-sample", type=int, default=8,
                     help="energies for the one-energy-at-a-time unfused FP64 reference point")
+    ap.add_argument("--serving-requests", type=int, default=1000,
+                    help="requests of the serving leg (a prefix of the Poisson stream; 0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=2000.0, help="offered load, requests/s")
+    ap.add_argument("--serving-budget-gib", type=float, default=96.0)
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "zgemm_traffic_32x256.json"))
     return ap.parse_args()
 
@@ -105,6 +111,86 @@ def cpu_baseline(model, energies, eta, budget_s):
         "kind": "port",
         "sample": f"{n} energies of the {g.n_blocks}x{g.block_sizes[0]} workload (numpy complex128 "
                   f"restatement of rgf.py, {cores} BLAS threads, {tn:.1f} s)",
+    }
+
+
+def torch_rgf_fp64(Ad, Au, Al, SLd, SLu, SGd, SGu, GRd, GLd, GLu, GGd, GGu):
+    """The reference selected solve (rgf.py:200-398, no OBC, no current) on torch tensors.
+
+    All arguments are (E, n, b, b) complex128 device views (uniform block size);
+    products go to rocBLAS batched ZGEMM and inverses to torch.linalg.inv, with
+    the energies of the call as the batch.  Outputs: G^R diag, G^< and G^> diag
+    (projected) and upper blocks.
+    """
+    import torch
+
+    H = lambda x: x.conj().transpose(-2, -1)  # noqa: E731
+    inv = torch.linalg.inv
+    N = Ad.shape[1]
+    xr, xl, xg = [None] * N, [None] * N, [None] * N
+    xr[0] = inv(Ad[:, 0])
+    xl[0] = xr[0] @ SLd[:, 0] @ H(xr[0])
+    xg[0] = xr[0] @ SGd[:, 0] @ H(xr[0])
+    for i in range(N - 1):
+        j = i + 1
+        a_ji = Al[:, i]
+        a_ji_xr = a_ji @ xr[i]
+        xr[j] = inv(Ad[:, j] - a_ji_xr @ Au[:, i])
+        for x, s_d, s_u in ((xl, SLd, SLu), (xg, SGd, SGu)):
+            t = a_ji_xr @ s_u[:, i]
+            x[j] = xr[j] @ (s_d[:, j] + a_ji @ x[i] @ H(a_ji) + H(t) - t) @ H(xr[j])
+    GLd[:, N - 1] = 0.5 * (xl[-1] - H(xl[-1]))
+    GGd[:, N - 1] = 0.5 * (xg[-1] - H(xg[-1]))
+    GRd[:, N - 1] = xr[-1]
+    for i in range(N - 2, -1, -1):
+        j = i + 1
+        a_ij, a_ji = Au[:, i], Al[:, i]
+        xr_ii_a_ij = xr[i] @ a_ij
+        xr_jj_a_ji = xr[j] @ a_ji
+        xr_ii_a_ij_xr_jj = xr_ii_a_ij @ xr[j]
+        m4 = xr_ii_a_ij @ xr_jj_a_ji
+        for x, s_u, gd, gu in ((xl, SLu, GLd, GLu), (xg, SGu, GGd, GGu)):
+            t1 = m4 @ x[i] - xr[i] @ s_u[:, i] @ H(xr_ii_a_ij_xr_jj)
+            t1 = t1 - H(t1)
+            t2 = xr_ii_a_ij @ x[j]
+            gu[:, i] = -t2 - x[i] @ H(xr_jj_a_ji) + xr[i] @ s_u[:, i] @ H(xr[j])
+            x[i] = x[i] + t2 @ H(xr_ii_a_ij) + t1
+            gd[:, i] = 0.5 * (x[i] - H(x[i]))
+        xr[i] = xr[i] + m4 @ xr[i]
+        GRd[:, i] = xr[i]
+
+
+def serving_leg(args, dev) -> dict:
+    """Continuous batching over 6 resident geometries, Poisson arrivals, adaptive precision."""
+    from quatrex_amd import serving as SV
+    from quatrex_amd import workloads as W
+
+    geos = SV.draw_geometries(6, W.GEOMETRY_SEED)
+    seeds = [W.GEOMETRY_SEED + 1 + k for k in range(len(geos))]
+    srv = SV.Server(geos, seeds, budget_bytes=int(args.serving_budget_gib * 2**30), capacity=8,
+                    slots_per_geometry=2, precision="adaptive", fallback_tol=1e-8, refine=1,
+                    device=dev)
+    try:
+        # warm every slot (graph capture, first launches) outside the measured stream
+        warm = [SV.Request(0.0, -1 - k, k % len(geos), 0.1 * k, 1e-3) for k in range(4 * len(geos))]
+        srv.run(warm, realtime=False)
+        reqs = SV.poisson_requests(args.serving_requests, args.serving_rate, len(geos), eta=1e-3)
+        res = srv.run(reqs, realtime=True)
+        st = SV.latency_stats(res)
+    finally:
+        srv.close()
+    cost = {f"{g.n_blocks}x{g.max_block}": round(flops_per_energy(g.n_blocks, g.max_block) / 1e12, 4)
+            for g in geos}
+    return {
+        "workload": f"{len(geos)} resident geometries (n_blocks x block: TFLOP per energy) {cost}, "
+                    f"Poisson arrivals at {args.serving_rate:g}/s, first {args.serving_requests} "
+                    f"requests of the stream, one energy point each, eta=1e-3",
+        "offered_rps": args.serving_rate,
+        "precision": "adaptive (mixed with FP64 check, FP64 fallback above 1e-8)",
+        "hbm_budget_gib": args.serving_budget_gib,
+        "slots": len(srv.slots),
+        "slot_capacity": sorted({s.capacity for s in srv.slots}),
+        **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
     }
 
 
@@ -234,6 +320,37 @@ def main():
                 "max_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs_fb.items()},
             },
         }
+    if args.torch_sample > 0:
+        # the reference algorithm on torch: rocBLAS batched ZGEMM + torch.linalg.inv, one
+        # energy batch of the reference's default max_batch_size (100)
+        ns = min(args.torch_sample, E)
+        nb, bs = args.n_blocks, args.block_size
+        v4 = lambda t, n: t[:ns].view(ns, n, bs, bs)  # noqa: E731
+        nc = min(10, ns)  # energies compared with this library's FP64 result
+        step()
+        ours = [X[:nc].clone() for X in (GR[0], GL[0], GG[0], GL[1], GG[1])]
+        tviews = (v4(A[0], nb), v4(A[1], nb - 1), v4(A[2], nb - 1), v4(sld, nb), v4(slu, nb - 1),
+                  v4(sgd, nb), v4(sgu, nb - 1), v4(GR[0], nb), v4(GL[0], nb), v4(GL[1], nb - 1),
+                  v4(GG[0], nb), v4(GG[1], nb - 1))
+        torch_rgf_fp64(*tviews)
+        torch.cuda.synchronize()
+        tt0 = time.perf_counter()
+        torch_rgf_fp64(*tviews)
+        torch.cuda.synchronize()
+        tt = time.perf_counter() - tt0
+        theirs = (GR[0], GL[0], GG[0], GL[1], GG[1])
+        agree = max(float((torch.linalg.norm((X[:nc] - Y).reshape(nc, -1), dim=-1)
+                           / torch.linalg.norm(Y.reshape(nc, -1), dim=-1)).max())
+                    for X, Y in zip(theirs, ours))
+        extra["torch_fp64"] = {
+            "value": round(ns / tt, 3),
+            "unit": "energy_points/s",
+            "definition": "reference algorithm (rgf.py selected_solve) on torch: rocBLAS batched "
+                          "ZGEMM + torch.linalg.inv, one batch of 100 energies",
+            "sample_energies": ns,
+            "batched_fp64_speedup": round(eps / (ns / tt), 2),
+            "max_rel_diff_vs_this_library": float(f"{agree:.3e}"),
+        }
     if args.plain_sample > 0:
         # plain FP64 GPU RGF: one energy per sweep, one product group per launch
         ns = min(args.plain_sample, E)
@@ -242,6 +359,7 @@ def main():
         pctx = S.Context(dev)
         pctx.set_option(1, 1)
         pctx.set_option(5, 1)
+        pctx.set_option(6, 0)
 
         def pstep():
             S.selected_solve(g, view(A), (sld[sl_], slu[sl_]), (sgd[sl_], sgu[sl_]), g_lesser=view(GL),
@@ -310,6 +428,12 @@ def main():
         },
         **extra,
     }
+
+    if args.serving_requests > 0:
+        del A, sld, sgd, slu, sgu, GR, GL, GG
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        out["serving"] = serving_leg(args, dev)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model, energies, args.eta, args.cpu_seconds)

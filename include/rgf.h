@@ -50,6 +50,9 @@ extern "C" {
 #define RGF_OPT_UNFUSED 5          /* 1: launch every product group on its own (the plain,
                                       ungrouped op sequence; with RGF_OPT_MAX_CHUNK = 1 this is
                                       the one-energy-at-a-time baseline)                      */
+#define RGF_OPT_OVERLAP 6          /* 1 (default): each block inversion runs on a second stream of
+                                      default priority, overlapped with the independent
+                                      lesser/greater products of its forward step              */
 #define RGF_OPT_PROFILE 4          /* 1: time every sweep GEMM launch and every block inversion
                                       with HIP events on the context stream (0: off)          */
 

@@ -25,6 +25,7 @@ RGF_OPT_WORKSPACE_BUDGET = 2
 RGF_OPT_TILE_CFG = 3
 RGF_OPT_PROFILE = 4
 RGF_OPT_UNFUSED = 5
+RGF_OPT_OVERLAP = 6
 RGF_OPT_TILE_FOR_BLOCK = 1000  # + block size (FP64 products)
 RGF_OPT_TILE_FOR_BLOCK_F32 = 3000  # + block size (FP32-compute products)
 RGF_NUM_TILE_CFGS = 4

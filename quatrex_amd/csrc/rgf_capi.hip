@@ -37,6 +37,9 @@ const char* status_name(int s) {
 // ---------------------------------------------------------------------------
 
 Ctx::~Ctx() {
+  if (side) { (void)hipStreamSynchronize(side); (void)hipStreamDestroy(side); }
+  if (ev_fork) (void)hipEventDestroy(ev_fork);
+  if (ev_join) (void)hipEventDestroy(ev_join);
   for (auto& t : timed) { event_pool.push_back(t.a); event_pool.push_back(t.b); }
   for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
   if (ws_owned && ws) (void)hipFree(ws);
@@ -191,19 +194,19 @@ hipEvent_t Ctx::take_event() {
   return e;
 }
 
-int Ctx::prof_begin(hipEvent_t* a) {
+int Ctx::prof_begin(hipEvent_t* a, hipStream_t s) {
   *a = nullptr;
   if (!profile) return RGF_OK;
   *a = take_event();
   if (!*a) return fail(RGF_EHIP, "hipEventCreate");
-  return hip(hipEventRecord(*a, stream), "hipEventRecord");
+  return hip(hipEventRecord(*a, s ? s : stream), "hipEventRecord");
 }
 
-int Ctx::prof_end(hipEvent_t a, double flops, int kind) {
+int Ctx::prof_end(hipEvent_t a, double flops, int kind, hipStream_t s) {
   if (!profile || !a) return RGF_OK;
   hipEvent_t b = take_event();
   if (!b) return fail(RGF_EHIP, "hipEventCreate");
-  int rc = hip(hipEventRecord(b, stream), "hipEventRecord");
+  int rc = hip(hipEventRecord(b, s ? s : stream), "hipEventRecord");
   if (rc) return rc;
   timed.push_back(Timed{a, b, flops, kind});
   if (timed.size() >= 4096) return prof_collect();
@@ -260,15 +263,34 @@ int Ctx::gemm(const std::vector<ZProblem>& probs, int nE, int b) {
   return RGF_OK;
 }
 
-int Ctx::inv(double2* S, int64_t sS, int n, int nE, Mat out, char* aux, int tag, int64_t e0) {
+int Ctx::inv(double2* S, int64_t sS, int n, int nE, Mat out, char* aux, int tag, int64_t e0,
+             hipStream_t on) {
+  hipStream_t s = on ? on : stream;
   hipEvent_t ev;
-  int rc = prof_begin(&ev);
+  int rc = prof_begin(&ev, s);
   if (rc) return rc;
-  rc = hip(launch_zinv(S, sS, n, nE, out.p, out.s, out.ld, aux, info_dev, tag, e0, tile_for(n),
-                       stream),
+  rc = hip(launch_zinv(S, sS, n, nE, out.p, out.s, out.ld, aux, info_dev, tag, e0, tile_for(n), s),
            "zinv launch");
   if (rc) return rc;
-  return prof_end(ev, 8.0 * n * (double)n * n * nE, 1);
+  return prof_end(ev, 8.0 * n * (double)n * n * nE, 1, s);
+}
+
+int Ctx::fork() {
+  if (!side) {
+    int rc = hip(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), "side stream");
+    if (!rc) rc = hip(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming), "event");
+    if (!rc) rc = hip(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming), "event");
+    if (rc) return rc;
+  }
+  int rc = hip(hipEventRecord(ev_fork, stream), "fork record");
+  if (rc) return rc;
+  return hip(hipStreamWaitEvent(side, ev_fork, 0), "fork wait");
+}
+
+int Ctx::join() {
+  int rc = hip(hipEventRecord(ev_join, side), "join record");
+  if (rc) return rc;
+  return hip(hipStreamWaitEvent(stream, ev_join, 0), "join wait");
 }
 
 }  // namespace qx
@@ -522,6 +544,14 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
                   PB(bj, bj).term(P, kOpN, SG.U(i), kOpN, bi).out(Qg).done()},
                  E, bj);
     if (rc) return rc;
+    // g_j = S^{-1} is independent of W: with overlap it runs on the side stream
+    const bool ovl = c->overlap;
+    if (ovl) {
+      rc = c->fork();
+      if (rc) return rc;
+      rc = c->inv(S.p, S.s, bj, E, GR.D(j), w.aux, j + 1, e0, c->side);
+      if (rc) return rc;
+    }
     // W = Y A_ji^H + Q^H - Q  [-> Sigma~_j]  + Sigma_jj + B_j
     rc = c->gemm({PB(bj, bj)
                       .term(Yl, kOpN, A.L(i), kOpC, bi)
@@ -543,7 +573,7 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
                       .done()},
                  E, bj);
     if (rc) return rc;
-    rc = c->inv(S.p, S.s, bj, E, GR.D(j), w.aux, j + 1, e0);
+    rc = ovl ? c->join() : c->inv(S.p, S.s, bj, E, GR.D(j), w.aux, j + 1, e0);
     if (rc) return rc;
     // Mixed precision: Newton steps g_j <- g_j + g_j (I - S64 g_j) with the Schur
     // complement recomputed in FP64 from the stored g_i (products in FP64,
@@ -955,6 +985,7 @@ int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value) {
       return RGF_OK;
     case RGF_OPT_PROFILE: ctx->profile = value != 0; return RGF_OK;
     case RGF_OPT_UNFUSED: ctx->unfused = value != 0; return RGF_OK;
+    case RGF_OPT_OVERLAP: ctx->overlap = value != 0; return RGF_OK;
     case RGF_OPT_TILE_CFG:
       if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
       ctx->tile_override = (int)value;

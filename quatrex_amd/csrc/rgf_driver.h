@@ -60,6 +60,13 @@ struct Ctx {
   int refine = 0;
   // one product group per launch (reference op order, no grouping)
   bool unfused = false;
+  // run each block inversion on a second stream, overlapped with the
+  // independent lesser/greater products of the same forward step
+  bool overlap = true;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int fork();   // side stream waits for the work queued so far on `stream`
+  int join();   // `stream` waits for the work queued so far on `side`
 
   // launch profiling (RGF_OPT_PROFILE)
   bool profile = false;
@@ -69,8 +76,8 @@ struct Ctx {
   int64_t prof_gemm_launches = 0, prof_inv_calls = 0;
   double prof_gemm_ms = 0, prof_gemm_flops = 0, prof_inv_ms = 0, prof_inv_flops = 0;
   hipEvent_t take_event();
-  int prof_begin(hipEvent_t* a);
-  int prof_end(hipEvent_t a, double flops, int kind);
+  int prof_begin(hipEvent_t* a, hipStream_t s = nullptr);
+  int prof_end(hipEvent_t a, double flops, int kind, hipStream_t s = nullptr);
   int prof_collect();
 
   virtual ~Ctx();
@@ -80,7 +87,8 @@ struct Ctx {
   int ensure_meta(size_t bytes);
   int tile_for(int b) const;
   int gemm(const std::vector<ZProblem>& probs, int nE, int b);
-  int inv(double2* S, int64_t sS, int n, int nE, Mat out, char* aux, int tag, int64_t e0);
+  int inv(double2* S, int64_t sS, int n, int nE, Mat out, char* aux, int tag, int64_t e0,
+          hipStream_t on = nullptr);
   int reset_info();
   int upload_geometry(const Geo& g);
   int64_t chunk_for(const Geo& g, int64_t nE, bool solve, bool current, bool have_gr);

@@ -28,6 +28,7 @@ from quatrex_amd.layout import Geometry
 DEVICE_SEED_BASE = 20260
 ENERGY_SEED = 777
 ARRIVAL_SEED = 4242
+GEOMETRY_SEED = 20265
 BOUNDARY_SIGMA = -0.5j
 
 
