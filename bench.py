@@ -61,6 +61,10 @@ def parse():
                     help="requests of the serving leg (a prefix of the Poisson stream; 0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=2000.0, help="offered load, requests/s")
     ap.add_argument("--serving-budget-gib", type=float, default=96.0)
+    ap.add_argument("--uneven-energies", type=int, default=2048,
+                    help="energies of the uneven-batching leg over 4 geometries (0 = skip)")
+    ap.add_argument("--uneven-capacity", type=int, default=64)
+    ap.add_argument("--uneven-budget-gib", type=float, default=200.0)
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "zgemm_traffic_32x256.json"))
     return ap.parse_args()
 
@@ -160,6 +164,107 @@ def torch_rgf_fp64(Ad, Au, Al, SLd, SLu, SGd, SGu, GRd, GLd, GLu, GGd, GGu):
         GRd[:, i] = xr[i]
 
 
+def torch_leg(args, model, energies, eps, dev) -> dict:
+    """The reference algorithm on torch (rocBLAS batched ZGEMM + torch.linalg.inv).
+
+    Runs one energy batch of the reference's default max_batch_size (100) on
+    its own inputs (the first energies of the workload) and compares the
+    result with this library's FP64 solve of the same inputs.
+    """
+    import torch
+
+    from quatrex_amd import solve as S
+    from quatrex_amd import workloads as W
+    from quatrex_amd.layout import Geometry
+
+    ns = min(args.torch_sample, len(energies))
+    nb, bs = args.n_blocks, args.block_size
+    g = Geometry.uniform(nb, bs)
+    A = W.system_matrix_torch(model, energies[:ns], args.eta, dev)
+    sld = torch.as_tensor(model.sl_diag, device=dev).expand(ns, -1).contiguous()
+    sgd = torch.as_tensor(model.sg_diag, device=dev).expand(ns, -1).contiguous()
+    slu = torch.zeros((ns, g.offdiag_size), dtype=torch.complex128, device=dev)
+    sgu = torch.zeros((ns, g.offdiag_size), dtype=torch.complex128, device=dev)
+    GR = S.packed_empty(g, ns, dev)
+    GL = S.packed_empty(g, ns, dev, parts=("d", "u"))
+    GG = S.packed_empty(g, ns, dev, parts=("d", "u"))
+    S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR)
+    S.default_context(dev).status()
+    ours = [X.clone() for X in (GR[0], GL[0], GG[0], GL[1], GG[1])]
+    v4 = lambda t, n: t.view(ns, n, bs, bs)  # noqa: E731
+    tviews = (v4(A[0], nb), v4(A[1], nb - 1), v4(A[2], nb - 1), v4(sld, nb), v4(slu, nb - 1),
+              v4(sgd, nb), v4(sgu, nb - 1), v4(GR[0], nb), v4(GL[0], nb), v4(GL[1], nb - 1),
+              v4(GG[0], nb), v4(GG[1], nb - 1))
+    torch_rgf_fp64(*tviews)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    torch_rgf_fp64(*tviews)
+    torch.cuda.synchronize()
+    tt = time.perf_counter() - t0
+    theirs = (GR[0], GL[0], GG[0], GL[1], GG[1])
+    agree = max(float((torch.linalg.norm((X - Y).reshape(ns, -1), dim=-1)
+                       / torch.linalg.norm(Y.reshape(ns, -1), dim=-1)).max())
+                for X, Y in zip(theirs, ours))
+    return {
+        "value": round(ns / tt, 3),
+        "unit": "energy_points/s",
+        "definition": "reference algorithm (rgf.py selected_solve) on torch: rocBLAS batched "
+                      "ZGEMM + torch.linalg.inv, one batch of 100 energies",
+        "sample_energies": ns,
+        "batched_fp64_speedup": round(eps / (ns / tt), 2),
+        "max_rel_diff_vs_this_library": float(f"{agree:.3e}"),
+    }
+
+
+def uneven_leg(args, dev) -> dict:
+    """Uneven batching: 2048 energies over 4 geometries, interleaved, FP64.
+
+    Each geometry is resident (H, boundary and scattering self-energies); a
+    slot per geometry assembles A on the device and replays a captured sweep
+    graph per batch, with the GEMM tile per block size picked by the
+    autotuner.  Batches of different geometries run concurrently on their own
+    streams; outputs are overwritten batch by batch.
+    """
+    from quatrex_amd import serving as SV
+    from quatrex_amd import workloads as W
+    from quatrex_amd.layout import Geometry
+
+    shapes = ((16, 128), (32, 256), (24, 384), (12, 512))
+    geos = [Geometry.uniform(nb, b) for nb, b in shapes]
+    seeds = [W.DEVICE_SEED_BASE + 10 + k for k in range(len(geos))]
+    per = args.uneven_energies // len(geos)
+    srv = SV.Server(geos, seeds, budget_bytes=int(args.uneven_budget_gib * 2**30),
+                    capacity=args.uneven_capacity, slots_per_geometry=1, precision="fp64",
+                    use_graphs=True, autotune=True, device=dev)
+    try:
+        cap = min(s.capacity for s in srv.slots)
+        # warm-up: one full batch per geometry captures the sweep graphs
+        warm = [SV.Request(0.0, -1 - k, k % len(geos), -0.5 + k / (4.0 * cap), 1e-3)
+                for k in range(len(geos) * cap)]
+        srv.run(warm, realtime=False)
+        rng = np.random.default_rng(W.ENERGY_SEED)
+        en = rng.uniform(-1.0, 1.0, size=per * len(geos))
+        reqs = [SV.Request(0.0, k, k % len(geos), float(en[k]), 1e-3) for k in range(len(en))]
+        res = srv.run(reqs, realtime=False)
+        st = SV.latency_stats(res)
+    finally:
+        srv.close()
+    elapsed = max(r.done for r in res) - min(r.arrival for r in res)
+    flops = sum(flops_per_energy(nb, b) for nb, b in shapes) * per
+    return {
+        "value": round(len(res) / elapsed, 3),
+        "unit": "energy_points/s",
+        "workload": f"{len(res)} energies, {per} per geometry (n_blocks, block) in {list(shapes)}, "
+                    "interleaved, eta=1e-3, FP64, G^R/G^</G^> selected blocks",
+        "achieved_tflops": round(flops / elapsed / 1e12, 3),
+        "frac_fp64_peak": round(flops / elapsed / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "batch_capacity": sorted({s.capacity for s in srv.slots}),
+        "batches": st["batches"],
+        "tiles": srv.tiles,
+        "elapsed_s": round(elapsed, 3),
+    }
+
+
 def serving_leg(args, dev) -> dict:
     """Continuous batching over 6 resident geometries, Poisson arrivals, adaptive precision."""
     from quatrex_amd import serving as SV
@@ -169,7 +274,7 @@ def serving_leg(args, dev) -> dict:
     seeds = [W.GEOMETRY_SEED + 1 + k for k in range(len(geos))]
     srv = SV.Server(geos, seeds, budget_bytes=int(args.serving_budget_gib * 2**30), capacity=8,
                     slots_per_geometry=2, precision="adaptive", fallback_tol=1e-8, refine=1,
-                    device=dev)
+                    autotune=True, device=dev)
     try:
         # warm every slot (graph capture, first launches) outside the measured stream
         warm = [SV.Request(0.0, -1 - k, k % len(geos), 0.1 * k, 1e-3) for k in range(4 * len(geos))]
@@ -190,6 +295,7 @@ def serving_leg(args, dev) -> dict:
         "hbm_budget_gib": args.serving_budget_gib,
         "slots": len(srv.slots),
         "slot_capacity": sorted({s.capacity for s in srv.slots}),
+        "tiles": srv.tiles,
         **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
     }
 
@@ -320,37 +426,6 @@ def main():
                 "max_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs_fb.items()},
             },
         }
-    if args.torch_sample > 0:
-        # the reference algorithm on torch: rocBLAS batched ZGEMM + torch.linalg.inv, one
-        # energy batch of the reference's default max_batch_size (100)
-        ns = min(args.torch_sample, E)
-        nb, bs = args.n_blocks, args.block_size
-        v4 = lambda t, n: t[:ns].view(ns, n, bs, bs)  # noqa: E731
-        nc = min(10, ns)  # energies compared with this library's FP64 result
-        step()
-        ours = [X[:nc].clone() for X in (GR[0], GL[0], GG[0], GL[1], GG[1])]
-        tviews = (v4(A[0], nb), v4(A[1], nb - 1), v4(A[2], nb - 1), v4(sld, nb), v4(slu, nb - 1),
-                  v4(sgd, nb), v4(sgu, nb - 1), v4(GR[0], nb), v4(GL[0], nb), v4(GL[1], nb - 1),
-                  v4(GG[0], nb), v4(GG[1], nb - 1))
-        torch_rgf_fp64(*tviews)
-        torch.cuda.synchronize()
-        tt0 = time.perf_counter()
-        torch_rgf_fp64(*tviews)
-        torch.cuda.synchronize()
-        tt = time.perf_counter() - tt0
-        theirs = (GR[0], GL[0], GG[0], GL[1], GG[1])
-        agree = max(float((torch.linalg.norm((X[:nc] - Y).reshape(nc, -1), dim=-1)
-                           / torch.linalg.norm(Y.reshape(nc, -1), dim=-1)).max())
-                    for X, Y in zip(theirs, ours))
-        extra["torch_fp64"] = {
-            "value": round(ns / tt, 3),
-            "unit": "energy_points/s",
-            "definition": "reference algorithm (rgf.py selected_solve) on torch: rocBLAS batched "
-                          "ZGEMM + torch.linalg.inv, one batch of 100 energies",
-            "sample_energies": ns,
-            "batched_fp64_speedup": round(eps / (ns / tt), 2),
-            "max_rel_diff_vs_this_library": float(f"{agree:.3e}"),
-        }
     if args.plain_sample > 0:
         # plain FP64 GPU RGF: one energy per sweep, one product group per launch
         ns = min(args.plain_sample, E)
@@ -429,10 +504,28 @@ def main():
         **extra,
     }
 
-    if args.serving_requests > 0:
-        del A, sld, sgd, slu, sgu, GR, GL, GG
-        torch.cuda.synchronize()
+    del A, sld, sgd, slu, sgu, GR, GL, GG
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if args.torch_sample > 0:
+        # torch's batched inverse on ROCm goes to hipBLAS/rocSOLVER by default;
+        # MAGMA is the alternative backend when that one fails
+        for backend in ("default", "magma"):
+            try:
+                if backend != "default":
+                    torch.backends.cuda.preferred_linalg_library(backend)
+                out["torch_fp64"] = torch_leg(args, model, energies, eps, dev)
+                out["torch_fp64"]["linalg_backend"] = backend
+                break
+            except RuntimeError as exc:
+                out["torch_fp64"] = {"error": f"{backend}: {str(exc)[:200]}"}
+            finally:
+                torch.cuda.synchronize()
+                torch.cuda.empty_cache()
+    if args.uneven_energies > 0:
+        out["uneven_batching"] = uneven_leg(args, dev)
         torch.cuda.empty_cache()
+    if args.serving_requests > 0:
         out["serving"] = serving_leg(args, dev)
 
     if not args.no_cpu_baseline:

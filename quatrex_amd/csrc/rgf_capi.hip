@@ -950,7 +950,8 @@ int rgf_create(int device, void* stream, rgf_ctx** out) {
     delete ctx;
     return RGF_ENOMEM;
   }
-  if (hipMemset(ctx->info_dev, 0, 4 * sizeof(int)) != hipSuccess) {
+  if (hipMemsetAsync(ctx->info_dev, 0, 4 * sizeof(int), ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess) {
     delete ctx;
     return RGF_EHIP;
   }
@@ -1018,10 +1019,12 @@ int rgf_status(rgf_ctx* ctx, int64_t* info) {
   int h[4] = {0, 0, 0, 0};
   hipError_t e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return ctx->hip(e, "hipStreamSynchronize");
-  e = hipMemcpy(h, ctx->info_dev, sizeof(h), hipMemcpyDeviceToHost);
+  // stream-ordered copies only: a legacy-stream call would wait on every
+  // blocking stream, including one another thread is capturing
+  e = hipMemcpyAsync(h, ctx->info_dev, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return ctx->hip(e, "read status");
-  e = hipMemset(ctx->info_dev, 0, sizeof(h));
-  if (e != hipSuccess) return ctx->hip(e, "reset status");
+  if (int rc = ctx->reset_info()) return rc;
   if (info) {
     info[0] = h[0] ? h[1] : -1;
     info[1] = h[0] ? h[0] - 1 : -1;
@@ -1320,7 +1323,9 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
     std::vector<int> sz(g.N);
     for (int i = 0; i < g.N; ++i) sz[i] = (int)g.bs[i];
     std::memcpy(host.data() + g.N * sizeof(int64_t), sz.data(), g.N * sizeof(int));
-    rc = ctx->hip(hipMemcpy(gr->tables, host.data(), tbytes, hipMemcpyHostToDevice), "graph tables copy");
+    rc = ctx->hip(hipMemcpyAsync(gr->tables, host.data(), tbytes, hipMemcpyHostToDevice, ctx->stream),
+                  "graph tables copy");
+    if (!rc) rc = ctx->hip(hipStreamSynchronize(ctx->stream), "graph tables sync");
     if (rc) { delete gr; return rc; }
   }
   Work w;
@@ -1350,14 +1355,25 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
   ctx->profile = false;
   ctx->prec32 = mixed;
   ctx->refine = mixed ? refine : 0;
+  // Capture is recorded on a private stream (the caller's stream may be the
+  // legacy NULL stream, which cannot be captured); the graph is launched on
+  // the context's stream.
   hipGraph_t graph = nullptr;
-  rc = ctx->hip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  hipStream_t saved_stream = ctx->stream;
+  hipStream_t cap = nullptr;
+  rc = ctx->hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "capture stream");
+  if (!rc) {
+    ctx->stream = cap;
+    rc = ctx->hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
+  }
   int rc_sweep = RGF_OK;
   if (!rc) {
     rc_sweep = ctx->reset_info();
     if (!rc_sweep) rc_sweep = sweep_chunk_solve(ctx, g, 0, (int)n_energies, io, w);
-    rc = ctx->hip(hipStreamEndCapture(ctx->stream, &graph), "end capture");
+    rc = ctx->hip(hipStreamEndCapture(cap, &graph), "end capture");
   }
+  ctx->stream = saved_stream;
+  if (cap) hipStreamDestroy(cap);
   ctx->offs_dev = saved_offs;
   ctx->sizes_dev = saved_sizes;
   ctx->profile = saved_prof;

@@ -138,12 +138,20 @@ class Slot:
     def _solve(self, batch: list, precision: str) -> np.ndarray:
         import torch
 
+        # Every torch call of a worker runs on the slot's stream: the thread's
+        # default (legacy) stream would synchronise with, and fail against, a
+        # sweep another slot is capturing into a graph.
+        with torch.cuda.stream(self.stream):
+            return self._solve_on_stream(batch, precision)
+
+    def _solve_on_stream(self, batch: list, precision: str) -> np.ndarray:
+        import torch
+
         n = len(batch)
         g = self.geometry
         res = self.server.resident[self.gid]
         zh = torch.tensor([r.energy + 1j * r.eta for r in batch], dtype=torch.complex128)
-        with torch.cuda.stream(self.stream):
-            self.z[:n].copy_(zh, non_blocking=False)
+        self.z[:n].copy_(zh, non_blocking=False)
         view = lambda T: tuple(t[:n] if t is not None else None for t in T)  # noqa: E731
         A = view(self.A)
         S.assemble(g, self.z[:n], res.h, a=A, sigma_b=res.sigma_b, ctx=self.ctx)
@@ -203,12 +211,15 @@ class Server:
     capacity : max energies per batch (per slot; reduced to fit the budget)
     slots_per_geometry : concurrent batches per geometry
     precision : "fp64", "mixed" or "adaptive"
+    autotune : time the GEMM tile configurations per block size at start-up
+        (cached on disk) and use the fastest in every slot
     """
 
     def __init__(self, geometries, seeds, budget_bytes: int, capacity: int = 8,
                  slots_per_geometry: int = 2, precision: str = "adaptive",
                  fallback_tol: float = 1e-8, refine: int = 1, retry_every: int = 8,
-                 keep_outputs: set | None = None, use_graphs: bool = True, device=None):
+                 keep_outputs: set | None = None, use_graphs: bool = True, autotune: bool = False,
+                 device=None):
         import torch
 
         if precision not in ("fp64", "mixed", "adaptive"):
@@ -243,10 +254,28 @@ class Server:
             for _ in range(slots_per_geometry):
                 self.slots.append(Slot(self, gid, cap, self.device))
             self.used += need
+        self.tiles: dict = {}
+        if autotune:
+            self._autotune(geometries, capacity)
         self.recent = [deque(maxlen=4) for _ in geometries]
         self.batches_since_retry = [0] * len(geometries)
         self._lock = threading.Condition()
         self._error = None
+
+    def _autotune(self, geometries, capacity: int) -> None:
+        """Pick the GEMM tile per block size (and precision) once; install it in every slot."""
+        from quatrex_amd import _lib
+        from quatrex_amd import autotune as AT
+
+        precs = (False,) if self.precision == "fp64" else (False, True)
+        sizes = sorted({int(b) for g in geometries for b in g.block_sizes})
+        for b in sizes:
+            for f32 in precs:
+                entry = AT.tune(self.slots[0].ctx, b, n_energies=capacity, f32=f32)
+                self.tiles[f"{'f32' if f32 else 'f64'}:{b}"] = entry["best_name"]
+                base = _lib.RGF_OPT_TILE_FOR_BLOCK_F32 if f32 else _lib.RGF_OPT_TILE_FOR_BLOCK
+                for slot in self.slots:
+                    slot.ctx.set_option(base + b, entry["best"])
 
     def _resident_bytes(self, g: Geometry) -> int:
         return 16 * (3 * g.diag_size + 3 * g.offdiag_size)

@@ -89,3 +89,33 @@ def test_realtime_poisson_stream(device):
         assert st["p99_ms"] >= st["p50_ms"] > 0
     finally:
         srv.close()
+
+
+def test_autotuned_uneven_batching_matches_alone(device, tmp_path, monkeypatch):
+    """Geometries of different block sizes in one server with the tile per block
+    size picked by the autotuner: every kept result equals the request solved
+    alone with the default tiles (to rounding: the tile shape may differ)."""
+    import torch
+
+    monkeypatch.setenv("QX_TILE_CACHE", str(tmp_path / "tiles.json"))
+    n = 24
+    reqs = [Request(0.0, k, k % 3, -0.9 + 0.07 * k) for k in range(n)]
+    keep = {0, 1, 2, 13, 22, 23}
+    srv = Server(GEOS, SEEDS, budget_bytes=8 << 30, capacity=4, slots_per_geometry=1,
+                 precision="fp64", keep_outputs=keep, autotune=True, device=device)
+    try:
+        assert set(srv.tiles) == {"f64:32", "f64:48", "f64:64"}
+        results = srv.run(reqs, realtime=False)
+        assert sorted(r.rid for r in results) == list(range(n))
+        for rid in keep:
+            req = reqs[rid]
+            ref = _alone(srv, req.geometry, req, "fp64")
+            for X, R in zip(srv.kept[rid], ref):
+                for part, rpart in zip(X, R):
+                    if part is None or rpart is None:
+                        continue
+                    r = rpart[0]
+                    err = float(torch.linalg.norm(part - r) / torch.linalg.norm(r))
+                    assert err <= 1e-12, (rid, err)
+    finally:
+        srv.close()
