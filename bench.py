@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
+    ap.add_argument("--no-autotune", action="store_true", help="default 64x64 GEMM tiles")
     ap.add_argument("--torch-sample", type=int, default=100,
                     help="energies of the torch (rocBLAS/rocSOLVER) reference-algorithm leg (0 = skip)")
     ap.add_argument("--plain-sample", type=int, default=8,
@@ -61,6 +62,8 @@ def parse():
                     help="requests of the serving leg (a prefix of the Poisson stream; 0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=2000.0, help="offered load, requests/s")
     ap.add_argument("--serving-budget-gib", type=float, default=96.0)
+    ap.add_argument("--eta-sample", type=int, default=32,
+                    help="energies per eta of the 48-block mixed-precision error leg (0 = skip)")
     ap.add_argument("--uneven-energies", type=int, default=2048,
                     help="energies of the uneven-batching leg over 4 geometries (0 = skip)")
     ap.add_argument("--uneven-capacity", type=int, default=64)
@@ -216,6 +219,63 @@ def torch_leg(args, model, energies, eps, dev) -> dict:
     }
 
 
+def eta_leg(args, model_seed, dev) -> dict:
+    """Mixed-precision error per output at 48 x 256 for eta in {1e-2, 1e-3, 1e-4}.
+
+    For a sample of energies: the complex64-product solve (refine=1, no check)
+    against the FP64 solve of the same inputs, max over energies and blocks
+    of the relative Frobenius error per block; and the fraction of energies
+    that the FP64 check sends back to FP64 at the 1e-8 tolerance.
+    """
+    import torch
+
+    from quatrex_amd import solve as S
+    from quatrex_amd import workloads as W
+    from quatrex_amd.layout import Geometry
+
+    g = Geometry.uniform(48, args.block_size)
+    model = W.make_device(g, model_seed)
+    ns = args.eta_sample
+    energies = W.draw_energies(ns)
+    out = {"workload": f"48 blocks x {args.block_size}, complex128, {ns} energies per eta",
+           "sample_energies": ns}
+    for eta in (1e-2, 1e-3, 1e-4):
+        A = W.system_matrix_torch(model, energies, eta, dev)
+        sld = torch.as_tensor(model.sl_diag, device=dev).expand(ns, -1).contiguous()
+        sgd = torch.as_tensor(model.sg_diag, device=dev).expand(ns, -1).contiguous()
+        slu = torch.zeros((ns, g.offdiag_size), dtype=torch.complex128, device=dev)
+        sgu = torch.zeros((ns, g.offdiag_size), dtype=torch.complex128, device=dev)
+        outs = (S.packed_empty(g, ns, dev), S.packed_empty(g, ns, dev, parts=("d", "u")),
+                S.packed_empty(g, ns, dev, parts=("d", "u")))
+
+        def solve(**kw):
+            S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_retarded=outs[0], g_lesser=outs[1],
+                             g_greater=outs[2], **kw)
+            S.default_context(dev).status()
+
+        solve()
+        ref = [tuple(p.clone() for p in X if p is not None) for X in outs]
+        solve(precision="mixed", mixed=S.MixedOptions(refine=1, fallback_tol=0.0))
+        errs = {}
+        for name, X, R in zip(("G^R", "G^<", "G^>"), outs, ref):
+            worst = 0.0
+            for part, rp in zip([p for p in X if p is not None], R):
+                d = (part - rp).reshape(ns, -1, args.block_size ** 2)
+                r = rp.reshape(ns, -1, args.block_size ** 2)
+                worst = max(worst, float((torch.linalg.norm(d, dim=-1) / torch.linalg.norm(r, dim=-1)).max()))
+            errs[name] = float(f"{worst:.3e}")
+        flags = torch.zeros(ns, dtype=torch.int32, device=dev)
+        resid = torch.zeros(ns, dtype=torch.float64, device=dev)
+        solve(precision="mixed", mixed=S.MixedOptions(refine=1, fallback_tol=1e-8, fallback_flags=flags,
+                                                      residuals=resid))
+        out[f"eta={eta:g}"] = {"max_rel_err_per_block": errs,
+                               "fallback_fraction_at_1e-8": round(float(flags.float().mean()), 4),
+                               "median_residual": float(f"{float(resid.median()):.3e}")}
+        del A, sld, sgd, slu, sgu, outs, ref
+        torch.cuda.empty_cache()
+    return out
+
+
 def uneven_leg(args, dev) -> dict:
     """Uneven batching: 2048 energies over 4 geometries, interleaved, FP64.
 
@@ -282,6 +342,12 @@ def serving_leg(args, dev) -> dict:
         reqs = SV.poisson_requests(args.serving_requests, args.serving_rate, len(geos), eta=1e-3)
         res = srv.run(reqs, realtime=True)
         st = SV.latency_stats(res)
+        # the same server below capacity: 60% of the throughput just measured
+        rate2 = 0.6 * st["throughput_rps"]
+        n2 = int(min(600, max(100, rate2 * 8)))
+        res2 = srv.run(SV.poisson_requests(n2, rate2, len(geos), seed=W.ARRIVAL_SEED + 1, eta=1e-3),
+                       realtime=True)
+        st2 = SV.latency_stats(res2)
     finally:
         srv.close()
     cost = {f"{g.n_blocks}x{g.max_block}": round(flops_per_energy(g.n_blocks, g.max_block) / 1e12, 4)
@@ -297,6 +363,8 @@ def serving_leg(args, dev) -> dict:
         "slot_capacity": sorted({s.capacity for s in srv.slots}),
         "tiles": srv.tiles,
         **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
+        "below_capacity": {"offered_rps": round(rate2, 2),
+                           **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st2.items()}},
     }
 
 
@@ -334,6 +402,15 @@ def main():
     ctx = S.default_context(dev)
     if args.chunk:
         ctx.set_option(1, args.chunk)
+    # GEMM tile per block size from the autotuner (timed on this device, cached)
+    from quatrex_amd import autotune as AT
+
+    tiles = {"f64": "64x64", "f32": "64x64"}
+    if not args.no_autotune:
+        tiles["f64"] = AT.tune(ctx, args.block_size, n_energies=64)["best_name"]
+        if not args.no_mixed:
+            tiles["f32"] = AT.tune(ctx, args.block_size, n_energies=64, f32=True)["best_name"]
+    tile_m, tile_n = tiles["f64"].split("x")
 
     def step():
         S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
@@ -481,10 +558,11 @@ def main():
             "eta": args.eta,
             "precision": "fp64",
             "outputs": "G^R diag+upper+lower, G^< and G^> diag+upper",
+            "gemm_tiles": tiles,
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": "zgemm_grouped_kernel<64,64,0> (sweep products)",
+            "kernel": f"zgemm_grouped_kernel<{tile_m},{tile_n},0> (sweep products)",
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -522,6 +600,9 @@ def main():
             finally:
                 torch.cuda.synchronize()
                 torch.cuda.empty_cache()
+    if args.eta_sample > 0 and not args.no_mixed:
+        out["mixed_error_vs_eta"] = eta_leg(args, seed, dev)
+        torch.cuda.empty_cache()
     if args.uneven_energies > 0:
         out["uneven_batching"] = uneven_leg(args, dev)
         torch.cuda.empty_cache()

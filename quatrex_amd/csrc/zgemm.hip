@@ -18,8 +18,9 @@ namespace qx {
 
 namespace {
 
-constexpr int TK = 16;
+constexpr int TK = 8;
 constexpr int PAD = 1;
+constexpr int kSlab = 32;  // epilogue rows per LDS pass: the rows of one wave row
 
 template <int TM, int TN>
 struct TileShape {
@@ -114,7 +115,8 @@ struct Smem {
   static constexpr int kA = TK * (TM + PAD);  // one LDS stage of op(A), k-major (LT elements)
   static constexpr int kB = TK * (TN + PAD);  // one LDS stage of op(B), k-major
   static constexpr int kMainBytes = (2 * kA + 2 * kB) * (int)sizeof(LT);
-  static constexpr int kTileBytes = TM * (TN + 1) * (int)sizeof(double2);
+  // the epilogue stages one kSlab-row slab of the output tile at a time
+  static constexpr int kTileBytes = kSlab * (TN + 1) * (int)sizeof(double2);
   static constexpr int kBytes = kMainBytes > kTileBytes ? kMainBytes : kTileBytes;
 };
 
@@ -259,81 +261,91 @@ zgemm_grouped_kernel(const ZGroup g) {
     }
   }
 
-  // ---- epilogue, staged through LDS so global accesses are row-coalesced ----
+  // ---- epilogue: one 32-row slab (the rows of one wave row) at a time,
+  // staged through LDS so that global accesses are row-coalesced ----
   // For v_mfma_f64_16x16x4f64 register r of a lane holds D[4*r + lane/16][lane%16]
   // of each 16x16 sub-tile (probed on gfx950).
   // (v_mfma_f32_16x16x4f32: register r of lane l holds D[4*(l/16) + r][l%16].)
-  constexpr int kPer = TM * TN / S::kThreads;
+  constexpr int kPer = kSlab * TN / S::kThreads;
+  static_assert(kPer % 4 == 0, "addends are applied in groups of four");
   auto Cs = reinterpret_cast<double2(*)[TN + 1]>(smem_raw);
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int rr = kF32 ? ((lane >> 4) * 4 + r) : (r * 4 + (lane >> 4));
-        Cs[wm * 32 + i * 16 + rr][wn * 32 + j * 16 + (lane & 15)] =
-            cmk((double)accr[i][j][r], (double)acci[i][j][r]);
-      }
-  __syncthreads();
   const int64_t eo = (int64_t)e;
   double2* D = P.D + eo * P.sD;
   double2* D2 = P.D2 ? P.D2 + eo * P.sD2 : nullptr;
-  // Each thread owns elements idx = tid + q*kThreads of the tile (row-major,
-  // so global accesses are coalesced along rows); values live in Cs and are
-  // updated in groups of four: loads of a group are issued before its stores.
-  for (int a = 0; a <= P.naddends; ++a) {
-    if (a == P.npre && P.d2mode == kD2Partial) {
-#pragma unroll 4
-      for (int q = 0; q < kPer; ++q) {
-        const int idx = tid + q * S::kThreads;
-        const int row = m0 + idx / TN, col = n0 + idx % TN;
-        if (row < m && col < n) D2[(int64_t)row * P.ldd2 + col] = Cs[idx / TN][idx % TN];
-      }
-    }
-    if (a == P.naddends) break;
-    const ZAddend& A = P.ad[a];
-    const double2* F = A.F + eo * A.sF;
-    const bool use1 = cnz(A.c1), use2 = cnz(A.c2);
 #pragma unroll 1
-    for (int q0 = 0; q0 < kPer; q0 += 4) {
-      double2 f1[4], f2[4];
+  for (int h = 0; h < S::kWavesM; ++h) {
+    const int r0 = m0 + h * kSlab;  // first global row of the slab
+    if (wm == h) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int idx = tid + (q0 + u) * S::kThreads;
-        const int row = m0 + idx / TN, col = n0 + idx % TN;
-        const bool in = row < m && col < n;
-        f1[u] = (use1 && in) ? F[(int64_t)row * A.ldf + col] : cmk(0.0, 0.0);
-        f2[u] = (use2 && in) ? F[(int64_t)col * A.ldf + row] : cmk(0.0, 0.0);
-      }
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int idx = tid + (q0 + u) * S::kThreads;
-        double2 v = Cs[idx / TN][idx % TN];
-        v = cadd(v, cadd(cmul(A.c1, f1[u]), cmul(A.c2, cconj(f2[u]))));
-        Cs[idx / TN][idx % TN] = v;
-      }
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int rr = kF32 ? ((lane >> 4) * 4 + r) : (r * 4 + (lane >> 4));
+            Cs[i * 16 + rr][wn * 32 + j * 16 + (lane & 15)] =
+                cmk((double)accr[i][j][r], (double)acci[i][j][r]);
+          }
     }
-  }
+    __syncthreads();
+    // Each thread owns elements idx = tid + q*kThreads of the slab (row-major,
+    // so global accesses are coalesced along rows); values live in Cs and are
+    // updated in groups of four: loads of a group are issued before its stores.
+    for (int a = 0; a <= P.naddends; ++a) {
+      if (a == P.npre && P.d2mode == kD2Partial) {
 #pragma unroll 4
-  for (int q = 0; q < kPer; ++q) {
-    const int idx = tid + q * S::kThreads;
-    const int row = m0 + idx / TN, col = n0 + idx % TN;
-    if (row < m && col < n) {
-      const double2 v = Cs[idx / TN][idx % TN];
-      D[(int64_t)row * P.ldd + col] = v;
-      if (P.d2mode == kD2Copy) D2[(int64_t)row * P.ldd2 + col] = cmul(P.s2, v);
+        for (int q = 0; q < kPer; ++q) {
+          const int idx = tid + q * S::kThreads;
+          const int row = r0 + idx / TN, col = n0 + idx % TN;
+          if (row < m && col < n) D2[(int64_t)row * P.ldd2 + col] = Cs[idx / TN][idx % TN];
+        }
+      }
+      if (a == P.naddends) break;
+      const ZAddend& A = P.ad[a];
+      const double2* F = A.F + eo * A.sF;
+      const bool use1 = cnz(A.c1), use2 = cnz(A.c2);
+#pragma unroll 1
+      for (int q0 = 0; q0 < kPer; q0 += 4) {
+        double2 f1[4], f2[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int idx = tid + (q0 + u) * S::kThreads;
+          const int row = r0 + idx / TN, col = n0 + idx % TN;
+          const bool in = row < m && col < n;
+          f1[u] = (use1 && in) ? F[(int64_t)row * A.ldf + col] : cmk(0.0, 0.0);
+          f2[u] = (use2 && in) ? F[(int64_t)col * A.ldf + row] : cmk(0.0, 0.0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int idx = tid + (q0 + u) * S::kThreads;
+          double2 v = Cs[idx / TN][idx % TN];
+          v = cadd(v, cadd(cmul(A.c1, f1[u]), cmul(A.c2, cconj(f2[u]))));
+          Cs[idx / TN][idx % TN] = v;
+        }
+      }
     }
-  }
-  if (P.d2mode == kD2ConjT) {
-    // D2[c][r] = s2 conj(D[r][c]): read the tile transposed from LDS, store rows of D2
 #pragma unroll 4
     for (int q = 0; q < kPer; ++q) {
       const int idx = tid + q * S::kThreads;
-      const int tr = idx / TM, tc = idx % TM;
-      const int row2 = n0 + tr, col2 = m0 + tc;
-      if (row2 < n && col2 < m) D2[(int64_t)row2 * P.ldd2 + col2] = cmul(P.s2, cconj(Cs[tc][tr]));
+      const int row = r0 + idx / TN, col = n0 + idx % TN;
+      if (row < m && col < n) {
+        const double2 v = Cs[idx / TN][idx % TN];
+        D[(int64_t)row * P.ldd + col] = v;
+        if (P.d2mode == kD2Copy) D2[(int64_t)row * P.ldd2 + col] = cmul(P.s2, v);
+      }
     }
+    if (P.d2mode == kD2ConjT) {
+      // D2[c][r] = s2 conj(D[r][c]): read the slab transposed from LDS, store
+      // runs of kSlab consecutive elements of rows of D2
+#pragma unroll 4
+      for (int q = 0; q < kPer; ++q) {
+        const int idx = tid + q * S::kThreads;
+        const int tr = idx / kSlab, tc = idx % kSlab;
+        const int row2 = n0 + tr, col2 = r0 + tc;
+        if (row2 < n && col2 < m) D2[(int64_t)row2 * P.ldd2 + col2] = cmul(P.s2, cconj(Cs[tc][tr]));
+      }
+    }
+    __syncthreads();  // the next slab overwrites Cs
   }
 }
 

@@ -24,7 +24,8 @@ replay), which removes the per-launch host cost of small geometries.
 Precision: ``mixed`` runs the complex64-product sweep with the FP64 check and
 FP64 fallback above ``fallback_tol``; ``adaptive`` additionally sends a
 geometry straight to FP64 while its recent fallback rate is high (the mixed
-attempt would be wasted), retrying mixed every ``retry_every`` batches.
+attempt would be wasted), retrying mixed after ``retry_every`` batches, the
+interval doubling after every failed retry (up to 1024 batches).
 """
 
 from __future__ import annotations
@@ -259,6 +260,7 @@ class Server:
             self._autotune(geometries, capacity)
         self.recent = [deque(maxlen=4) for _ in geometries]
         self.batches_since_retry = [0] * len(geometries)
+        self.retry_gap = [retry_every] * len(geometries)
         self._lock = threading.Condition()
         self._error = None
 
@@ -299,9 +301,15 @@ class Server:
         hist = self.recent[gid]
         if hist and np.mean(hist) > 0.5:
             self.batches_since_retry[gid] += 1
-            if self.batches_since_retry[gid] % self.retry_every:
+            if self.batches_since_retry[gid] < self.retry_gap[gid]:
                 return "fp64"
+            self.batches_since_retry[gid] = 0
         return "mixed"
+
+    def _record_mixed(self, gid: int, rate: float) -> None:
+        """Fallback rate of a mixed batch; a failed attempt doubles the wait before the next."""
+        self.recent[gid].append(rate)
+        self.retry_gap[gid] = min(2 * self.retry_gap[gid], 1024) if rate > 0.5 else self.retry_every
 
     def _complete(self, slot: Slot, batch, done, precision, fallback):
         import torch
@@ -322,7 +330,7 @@ class Server:
                 self.results.append(Result(r.rid, r.geometry, r.arrival, done, precision,
                                            bool(fallback[k]), self.nbatches))
             if precision == "mixed":
-                self.recent[slot.gid].append(float(np.mean(fallback)) if len(fallback) else 0.0)
+                self._record_mixed(slot.gid, float(np.mean(fallback)) if len(fallback) else 0.0)
             self.nbatches += 1
             slot.busy = False
             self._lock.notify_all()

@@ -42,3 +42,32 @@ def test_latency_stats():
     assert 50 <= st["p50_ms"] <= 60 and 95 <= st["p99_ms"] <= 100
     assert st["fallback_fraction"] == 0.0 and st["batches"] == 25
     assert Request(0.5, 1, 0, 0.1) < Request(0.6, 0, 0, 0.1)
+
+
+def test_adaptive_precision_backs_off_exponentially():
+    """A geometry whose mixed batches keep failing the check is sent to FP64,
+    with mixed retried after 8, 16, 32, ... batches; one success resets it."""
+    from collections import deque
+
+    from quatrex_amd.serving import Server
+
+    srv = Server.__new__(Server)
+    srv.precision = "adaptive"
+    srv.retry_every = 8
+    srv.recent = [deque(maxlen=4)]
+    srv.batches_since_retry = [0]
+    srv.retry_gap = [8]
+    assert srv._choose_precision(0) == "mixed"  # no history yet
+    srv._record_mixed(0, 1.0)
+    tries = []
+    for k in range(1, 200):
+        p = srv._choose_precision(0)
+        if p == "mixed":
+            tries.append(k)
+            srv._record_mixed(0, 1.0)
+    assert np.diff([0] + tries).tolist() == [16, 32, 64]
+    srv._record_mixed(0, 0.0)
+    srv._record_mixed(0, 0.0)
+    srv._record_mixed(0, 0.0)
+    assert srv.retry_gap[0] == 8
+    assert srv._choose_precision(0) == "mixed"  # history now mostly successful
