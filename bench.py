@@ -58,8 +58,10 @@ def parse():
                     help="energies of the torch (rocBLAS/rocSOLVER) reference-algorithm leg (0 = skip)")
     ap.add_argument("--plain-sample", type=int, default=8,
                     help="energies for the one-energy-at-a-time unfused FP64 reference point")
-    ap.add_argument("--serving-requests", type=int, default=1000,
-                    help="requests of the serving leg (a prefix of the Poisson stream; 0 = skip)")
+    ap.add_argument("--serving-requests", type=int, default=0,
+                    help="requests of the serving leg (a prefix of the Poisson stream; 0 = skip). "
+                         "Off by default: its process ended with SIGSEGV in 2 of its last 3 runs "
+                         "and the cause is not found yet")
     ap.add_argument("--serving-rate", type=float, default=2000.0, help="offered load, requests/s")
     ap.add_argument("--serving-budget-gib", type=float, default=96.0)
     ap.add_argument("--eta-sample", type=int, default=32,
@@ -68,6 +70,10 @@ def parse():
                     help="energies of the uneven-batching leg over 4 geometries (0 = skip)")
     ap.add_argument("--uneven-capacity", type=int, default=64)
     ap.add_argument("--uneven-budget-gib", type=float, default=200.0)
+    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving"), default=None,
+                    help=argparse.SUPPRESS)
+    ap.add_argument("--main-eps", type=float, default=0.0, help=argparse.SUPPRESS)
+    ap.add_argument("--leg-timeout", type=float, default=300.0)
     ap.add_argument("--traffic-file", default=str(ROOT / "profiles" / "zgemm_traffic_32x256.json"))
     return ap.parse_args()
 
@@ -109,7 +115,7 @@ def cpu_baseline(model, energies, eta, budget_s):
     limiter = threadpool_limits(limits=cores) if threadpool_limits else contextlib.nullcontext()
     with limiter:
         t1 = run(1)
-        n = int(max(1, min(16, budget_s // max(t1, 1e-3))))
+        n = int(max(1, min(64, budget_s // max(t1, 1e-3))))
         tn = run(n) if n > 1 else t1
     return {
         "value": n / tn,
@@ -368,12 +374,66 @@ def serving_leg(args, dev) -> dict:
     }
 
 
+LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_sample",
+            "uneven_energies", "uneven_capacity", "uneven_budget_gib", "serving_requests",
+            "serving_rate", "serving_budget_gib")
+
+
+def run_leg(leg: str, args, eps: float) -> dict:
+    """Run one secondary leg in a child process; its JSON line, or the error."""
+    import subprocess
+
+    cmd = [sys.executable, "-u", str(Path(__file__).resolve()), "--only-leg", leg,
+           "--main-eps", repr(eps)]
+    for name in LEG_ARGS:
+        cmd += ["--" + name.replace("_", "-"), str(getattr(args, name))]
+    try:
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=args.leg_timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {args.leg_timeout} s"}
+    lines = [ln for ln in res.stdout.splitlines() if ln.startswith("{")]
+    if res.returncode != 0 or not lines:
+        tail = (res.stderr or "").strip().splitlines()[-3:]
+        return {"error": f"exit code {res.returncode}", "stderr_tail": tail}
+    return json.loads(lines[-1])
+
+
+def leg_main(args) -> None:
+    """Child-process entry of one secondary leg: prints its JSON line."""
+    import torch
+
+    from quatrex_amd import workloads as W
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    seed = W.DEVICE_SEED_BASE + 2
+    if args.only_leg == "torch":
+        # torch's default batched inverse on ROCm (hipBLAS getrfBatched) fails with
+        # an allocation error inside its fixed rocBLAS workspace; MAGMA is used
+        torch.backends.cuda.preferred_linalg_library("magma")
+        from quatrex_amd.layout import Geometry
+
+        model = W.make_device(Geometry.uniform(args.n_blocks, args.block_size), seed)
+        res = torch_leg(args, model, W.draw_energies(args.energies), args.main_eps, dev)
+        res["linalg_backend"] = "magma"
+    elif args.only_leg == "eta":
+        res = eta_leg(args, seed, dev)
+    elif args.only_leg == "uneven":
+        res = uneven_leg(args, dev)
+    else:
+        res = serving_leg(args, dev)
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.only_leg:
+        leg_main(args)
+        return
 
     import torch
 
@@ -585,29 +645,22 @@ def main():
     del A, sld, sgd, slu, sgu, GR, GL, GG
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
+    # The secondary legs run one child process each: a failure there (a third
+    # party library included) is reported in the JSON instead of ending the run.
+    legs = []
     if args.torch_sample > 0:
-        # torch's batched inverse on ROCm goes to hipBLAS/rocSOLVER by default;
-        # MAGMA is the alternative backend when that one fails
-        for backend in ("default", "magma"):
-            try:
-                if backend != "default":
-                    torch.backends.cuda.preferred_linalg_library(backend)
-                out["torch_fp64"] = torch_leg(args, model, energies, eps, dev)
-                out["torch_fp64"]["linalg_backend"] = backend
-                break
-            except RuntimeError as exc:
-                out["torch_fp64"] = {"error": f"{backend}: {str(exc)[:200]}"}
-            finally:
-                torch.cuda.synchronize()
-                torch.cuda.empty_cache()
+        legs.append(("torch_fp64", "torch"))
     if args.eta_sample > 0 and not args.no_mixed:
-        out["mixed_error_vs_eta"] = eta_leg(args, seed, dev)
-        torch.cuda.empty_cache()
+        legs.append(("mixed_error_vs_eta", "eta"))
     if args.uneven_energies > 0:
-        out["uneven_batching"] = uneven_leg(args, dev)
-        torch.cuda.empty_cache()
+        legs.append(("uneven_batching", "uneven"))
     if args.serving_requests > 0:
-        out["serving"] = serving_leg(args, dev)
+        legs.append(("serving", "serving"))
+    else:
+        out["serving"] = {"skipped": "off by default: the serving leg's process ended with SIGSEGV "
+                                     "in 2 of its last 3 runs; cause not found yet"}
+    for key, leg in legs:
+        out[key] = run_leg(leg, args, eps)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model, energies, args.eta, args.cpu_seconds)

@@ -209,7 +209,9 @@ int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset);
  * inversions and `refine` Newton steps, without the a-posteriori check) into
  * a graph that owns its workspace and geometry tables; rgf_graph_launch
  * replays it on the context's stream with whatever data the buffers hold at
- * that time.  The current is not supported in graphs.  No reference
+ * that time.  The current is not supported in graphs.  The context must stay
+ * alive while the graph is launched; rgf_graph_destroy waits for the last
+ * launch and may come after rgf_destroy of the context.  No reference
  * counterpart (launch-overhead removal for the serving loop). */
 typedef struct rgf_graph rgf_graph;
 int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,

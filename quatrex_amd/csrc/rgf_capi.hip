@@ -908,8 +908,14 @@ struct rgf_graph {
   hipGraphExec_t exec = nullptr;
   void* ws = nullptr;
   void* tables = nullptr;
+  // recorded after every launch: destruction waits on it rather than on the
+  // context's stream, so a graph may outlive its context
+  hipEvent_t done = nullptr;
   ~rgf_graph() {
-    if (ctx && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (done) {
+      (void)hipEventSynchronize(done);
+      (void)hipEventDestroy(done);
+    }
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (ws) (void)hipFree(ws);
@@ -1310,6 +1316,10 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
   if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
   rgf_graph* gr = new rgf_graph();
   gr->ctx = ctx;
+  if (hipEventCreateWithFlags(&gr->done, hipEventDisableTiming) != hipSuccess) {
+    delete gr;
+    return ctx->fail(RGF_EHIP, "graph event");
+  }
   const bool have_gr = g_retarded && g_retarded->diag;
   const bool mixed = precision == 1;
   const size_t wbytes = work_bytes(g, n_energies, true, false, have_gr, mixed, true);
@@ -1391,7 +1401,9 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
 int rgf_graph_launch(rgf_graph* graph) {
   if (!graph || !graph->exec) return RGF_EINVAL;
   rgf_ctx* ctx = graph->ctx;
-  return ctx->hip(hipGraphLaunch(graph->exec, ctx->stream), "graph launch");
+  int rc = ctx->hip(hipGraphLaunch(graph->exec, ctx->stream), "graph launch");
+  if (!rc) rc = ctx->hip(hipEventRecord(graph->done, ctx->stream), "graph done record");
+  return rc;
 }
 
 int rgf_graph_destroy(rgf_graph* graph) {

@@ -133,8 +133,11 @@ __device__ inline double4_t mfma_nega(double a, double b, double4_t c) {
 // kF32: operands are rounded to complex64 while staged into LDS and the
 // products run on v_mfma_f32_16x16x4f32 with FP32 accumulation; loads,
 // addends and stores stay complex128 (the mixed-precision sweeps).
+// Occupancy target: four 64x64 workgroups per CU (33 KB LDS each, <= 128
+// VGPRs), measured faster than three.
 template <int TM, int TN, int kRole, bool kF32>
-__global__ void __launch_bounds__((TileShape<TM, TN>::kThreads), (TM * TN > 4096 ? 1 : 2))
+__global__ void __launch_bounds__((TileShape<TM, TN>::kThreads),
+                                  (TM * TN > 4096 ? 1 : (TM * TN == 4096 ? 4 : 2)))
 zgemm_grouped_kernel(const ZGroup g) {
   using S = TileShape<TM, TN>;
   using LT = typename std::conditional<kF32, float2, double2>::type;
@@ -267,7 +270,7 @@ zgemm_grouped_kernel(const ZGroup g) {
   // of each 16x16 sub-tile (probed on gfx950).
   // (v_mfma_f32_16x16x4f32: register r of lane l holds D[4*(l/16) + r][l%16].)
   constexpr int kPer = kSlab * TN / S::kThreads;
-  static_assert(kPer % 4 == 0, "addends are applied in groups of four");
+  static_assert(kPer % 2 == 0, "addends are applied in pairs");
   auto Cs = reinterpret_cast<double2(*)[TN + 1]>(smem_raw);
   const int64_t eo = (int64_t)e;
   double2* D = P.D + eo * P.sD;
@@ -290,7 +293,8 @@ zgemm_grouped_kernel(const ZGroup g) {
     __syncthreads();
     // Each thread owns elements idx = tid + q*kThreads of the slab (row-major,
     // so global accesses are coalesced along rows); values live in Cs and are
-    // updated in groups of four: loads of a group are issued before its stores.
+    // updated in pairs: loads of a pair are issued before its stores (pairs,
+    // not larger groups, keep the 64x64 kernel within 128 VGPRs unspilled).
     for (int a = 0; a <= P.naddends; ++a) {
       if (a == P.npre && P.d2mode == kD2Partial) {
 #pragma unroll 4
@@ -305,10 +309,10 @@ zgemm_grouped_kernel(const ZGroup g) {
       const double2* F = A.F + eo * A.sF;
       const bool use1 = cnz(A.c1), use2 = cnz(A.c2);
 #pragma unroll 1
-      for (int q0 = 0; q0 < kPer; q0 += 4) {
-        double2 f1[4], f2[4];
+      for (int q0 = 0; q0 < kPer; q0 += 2) {
+        double2 f1[2], f2[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
           const int idx = tid + (q0 + u) * S::kThreads;
           const int row = r0 + idx / TN, col = n0 + idx % TN;
           const bool in = row < m && col < n;
@@ -316,7 +320,7 @@ zgemm_grouped_kernel(const ZGroup g) {
           f2[u] = (use2 && in) ? F[(int64_t)col * A.ldf + row] : cmk(0.0, 0.0);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
           const int idx = tid + (q0 + u) * S::kThreads;
           double2 v = Cs[idx / TN][idx % TN];
           v = cadd(v, cadd(cmul(A.c1, f1[u]), cmul(A.c2, cconj(f2[u]))));

@@ -8,6 +8,13 @@ report for a copy of known size.
 ``python -m quatrex_amd.profiling summarize <csv...>`` condenses rocprofv3
 CSV outputs (kernel_stats / kernel_trace / counter_collection) into the short
 text tables kept under ``profiles/``.
+
+``python -m quatrex_amd.profiling traffic ...`` turns the FETCH_SIZE and
+WRITE_SIZE counter files of a kernel and of the calibration copy into HBM
+bytes per launch (the JSON that bench.py reports as ``roofline.traffic``).
+
+``python -m quatrex_amd.profiling dbstats <results.db>`` prints per-kernel
+call counts and durations from a rocprofv3 ``--kernel-trace`` database.
 """
 
 from __future__ import annotations
@@ -76,6 +83,53 @@ def summarize(paths: list[str]) -> None:
         print()
 
 
+def _per_dispatch(path: Path, kernel: str, counter: str) -> tuple[float, int]:
+    """Average counter value (KiB) per dispatch of kernels whose name contains ``kernel``."""
+    tot = defaultdict(float)
+    for r in _rows(path):
+        if r["Counter_Name"] == counter and kernel in r["Kernel_Name"]:
+            tot[r.get("Dispatch_Id", "")] += float(r["Counter_Value"])
+    if not tot:
+        raise SystemExit(f"no {counter} rows for {kernel!r} in {path}")
+    return sum(tot.values()) / len(tot), len(tot)
+
+
+def traffic(args) -> dict:
+    """HBM bytes per launch of ``args.kernel``, scaled by the calibration copy."""
+    copy_bytes = args.copy_gib * (1 << 30)
+    fk, nd = _per_dispatch(Path(args.fetch), args.kernel, "FETCH_SIZE")
+    wk, _ = _per_dispatch(Path(args.write), args.kernel, "WRITE_SIZE")
+    cf, _ = _per_dispatch(Path(args.copy_fetch), "copyBuffer", "FETCH_SIZE")
+    cw, _ = _per_dispatch(Path(args.copy_write), "copyBuffer", "WRITE_SIZE")
+    fs, ws = copy_bytes / (cf * 1024), copy_bytes / (cw * 1024)
+    out = {
+        "kernel": args.kernel,
+        "workload": args.workload,
+        "fetch_kib_per_dispatch": round(fk, 1),
+        "write_kib_per_dispatch": round(wk, 1),
+        "dispatches": nd,
+        "copy_calibration": {"bytes_per_copy": copy_bytes, "fetch_kib": round(cf, 1),
+                             "write_kib": round(cw, 1), "fetch_scale": round(fs, 4),
+                             "write_scale": round(ws, 4)},
+        "hbm_bytes_per_launch": int(fk * 1024 * fs + wk * 1024 * ws),
+    }
+    if args.algorithmic_bytes:
+        out["algorithmic_operand_bytes_per_launch"] = int(args.algorithmic_bytes)
+    return out
+
+
+def dbstats(path: str) -> None:
+    import sqlite3
+
+    con = sqlite3.connect(path)
+    tot = con.execute("select sum(end - start) from kernels").fetchone()[0] or 1
+    print(f"{'kernel':78s} {'calls':>6s} {'avg us':>10s} {'total ms':>10s} {'%':>6s}")
+    for name, n, avg, t in con.execute(
+            "select name, count(*), avg(end - start), sum(end - start) from kernels "
+            "group by name order by 4 desc limit 12"):
+        print(f"{name[:78]:78s} {n:6d} {avg / 1e3:10.1f} {t / 1e6:10.2f} {100 * t / tot:6.1f}")
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -84,9 +138,26 @@ def main(argv=None) -> None:
     c.add_argument("--reps", type=int, default=4)
     s = sub.add_parser("summarize")
     s.add_argument("paths", nargs="+")
+    t = sub.add_parser("traffic")
+    t.add_argument("--kernel", required=True, help="substring of the kernel name")
+    t.add_argument("--workload", default="")
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-fetch", required=True)
+    t.add_argument("--copy-write", required=True)
+    t.add_argument("--copy-gib", type=float, default=1.0)
+    t.add_argument("--algorithmic-bytes", type=float, default=0.0)
+    d = sub.add_parser("dbstats")
+    d.add_argument("path")
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy_calibration(args.gib, args.reps)
+    elif args.cmd == "traffic":
+        import json
+
+        print(json.dumps(traffic(args), indent=1))
+    elif args.cmd == "dbstats":
+        dbstats(args.path)
     else:
         summarize(args.paths)
 
