@@ -392,15 +392,22 @@ def run_leg(leg: str, args, eps: float) -> dict:
     except subprocess.TimeoutExpired:
         return {"error": f"timed out after {args.leg_timeout} s"}
     lines = [ln for ln in res.stdout.splitlines() if ln.startswith("{")]
-    if res.returncode != 0 or not lines:
-        tail = (res.stderr or "").strip().splitlines()[-3:]
+    if not lines:
+        tail = (res.stderr or "").strip().splitlines()[-12:]
         return {"error": f"exit code {res.returncode}", "stderr_tail": tail}
-    return json.loads(lines[-1])
+    out = json.loads(lines[-1])
+    if res.returncode != 0:  # the leg finished, the process did not end cleanly
+        out["process_exit_code"] = res.returncode
+    return out
 
 
 def leg_main(args) -> None:
     """Child-process entry of one secondary leg: prints its JSON line."""
+    import faulthandler
+
     import torch
+
+    faulthandler.enable(all_threads=True)  # a native crash prints every thread's stack
 
     from quatrex_amd import workloads as W
 

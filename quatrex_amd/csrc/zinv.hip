@@ -182,14 +182,27 @@ zinv_panel_kernel(double2* S, int64_t sS, int n, int k, int nbe, char* aux, int6
     }
     __syncthreads();
   }
-  // move the pivot rows' non-panel columns to Wp, leaving zeros in S
+  // move the pivot rows' non-panel columns to Wp, leaving zeros in S.  Every
+  // thread owns one column (rows >= n); the values of 16 pivot rows are loaded
+  // before any of them is stored, so the loads are not serialised behind the
+  // stores (the pointers may alias as far as the compiler knows).
   if (nbe < n) {
-    for (int q = 0; q < nbe; ++q) {
-      double2* row = A + (int64_t)sh.s_piv[q] * n;
-      for (int c = tid; c < n; c += rows) {
-        if (c < k || c >= k + nbe) {
-          Wp[(int64_t)q * n + c] = row[c];
-          row[c] = cmk(0.0, 0.0);
+    const int c = tid;
+    const bool outside = c < n && (c < k || c >= k + nbe);
+#pragma unroll 1
+    for (int q0 = 0; q0 < nbe; q0 += 16) {
+      double2 v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int q = q0 + u;
+        v[u] = (outside && q < nbe) ? A[(int64_t)sh.s_piv[q] * n + c] : cmk(0.0, 0.0);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int q = q0 + u;
+        if (outside && q < nbe) {
+          Wp[(int64_t)q * n + c] = v[u];
+          A[(int64_t)sh.s_piv[q] * n + c] = cmk(0.0, 0.0);
         }
       }
     }

@@ -15,6 +15,10 @@ bytes per launch (the JSON that bench.py reports as ``roofline.traffic``).
 
 ``python -m quatrex_amd.profiling dbstats <results.db>`` prints per-kernel
 call counts and durations from a rocprofv3 ``--kernel-trace`` database.
+
+``python -m quatrex_amd.profiling inv --n N --energies E`` times the batched
+complex inverse (``rgf_inv_z``) of E diagonally dominant N x N matrices and
+checks the first result against numpy.
 """
 
 from __future__ import annotations
@@ -38,6 +42,34 @@ def copy_calibration(gib: float, reps: int) -> None:
         b.copy_(a)
     torch.cuda.synchronize()
     print(f"copied {reps} x {n * 16} bytes")
+
+
+def time_inverse(n: int, energies: int, reps: int) -> None:
+    import numpy as np
+    import torch
+
+    from quatrex_amd import solve as S
+
+    dev = torch.device("cuda", 0)
+    a = torch.complex(torch.rand(energies, n, n, dtype=torch.float64, device=dev) - 0.5,
+                      torch.rand(energies, n, n, dtype=torch.float64, device=dev) - 0.5)
+    a += 4.0 * torch.eye(n, dtype=torch.complex128, device=dev)
+    out = torch.empty_like(a)
+    S.inv(a, out=out)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for _ in range(reps):
+        t0.record()
+        S.inv(a, out=out)
+        t1.record()
+        t1.synchronize()
+        best = min(best, t0.elapsed_time(t1))
+    x = out[0].cpu().numpy()
+    err = np.abs(a[0].cpu().numpy() @ x - np.eye(n)).max()
+    flops = 8.0 * n ** 3 * energies
+    print(f"inverse n={n} energies={energies}: {best:.3f} ms, {flops / best / 1e9:.2f} TFLOP/s "
+          f"(8 n^3 per matrix), max |A X - I| = {err:.2e}")
 
 
 def _rows(path: Path):
@@ -149,6 +181,10 @@ def main(argv=None) -> None:
     t.add_argument("--algorithmic-bytes", type=float, default=0.0)
     d = sub.add_parser("dbstats")
     d.add_argument("path")
+    v = sub.add_parser("inv")
+    v.add_argument("--n", type=int, default=256)
+    v.add_argument("--energies", type=int, default=512)
+    v.add_argument("--reps", type=int, default=4)
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy_calibration(args.gib, args.reps)
@@ -158,6 +194,8 @@ def main(argv=None) -> None:
         print(json.dumps(traffic(args), indent=1))
     elif args.cmd == "dbstats":
         dbstats(args.path)
+    elif args.cmd == "inv":
+        time_inverse(args.n, args.energies, args.reps)
     else:
         summarize(args.paths)
 
