@@ -203,6 +203,37 @@ typedef struct rgf_profile {
 } rgf_profile;
 int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset);
 
+/* Tridiagonal blocks of a block-sparse stack <-> the packed layout of rgf_bt.
+ * Replaces the per-block densify / sparsify of DSDBCOO._get_block /
+ * _set_block (src/qttools/datastructures/dsdbcoo.py:333-412, 466-521) and
+ * DSDBCSR._get_block / _set_block (dsdbcsr.py:285-345, 420-472), for the
+ * blocks RGF reads and writes (a.blocks[i,i], [i,i+1], [i+1,i]).
+ * data: complex128, n_energies rows of data_stride elements (nnz per energy).
+ * The map (device arrays, built once per sparsity structure) has one part per
+ * output part (0 diag, 1 upper, 2 lower): entry k moves data[e][src[k]] to
+ * part[e][dst[k]], dst being the packed offset inside that part.  op[k] flags:
+ * 1 = mirror image of a stored entry under symmetric storage (densify only,
+ * value op(v)); 2 = stored diagonal element of a symmetric diagonal block
+ * (densifies to (v + op(v)) / 2); op(v) = s * conj(v), or s * v with flag 8,
+ * s = -1 with flag 4.  rgf_densify_z zeroes every non-NULL output part first
+ * (blocks absent from the pattern read as zeros); rgf_sparsify_z writes the
+ * stored entries only (values outside the pattern are dropped, as in the
+ * reference). */
+typedef struct rgf_block_map_part {
+  int64_t n;
+  const int64_t* src;
+  const int64_t* dst;
+  const uint8_t* op;
+} rgf_block_map_part;
+typedef struct rgf_block_map {
+  rgf_block_map_part part[3];
+} rgf_block_map;
+int rgf_densify_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies, const void* data,
+                  int64_t data_stride, const rgf_block_map* map, rgf_bt* out);
+int rgf_sparsify_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                   const rgf_bt_const* in, const rgf_block_map* map, void* data,
+                   int64_t data_stride);
+
 /* HIP graph of a complete selected-solve sweep for fixed buffers.
  * rgf_graph_capture_solve records every launch of one sweep over n_energies
  * energies (precision 0: FP64; 1: FP32-compute products with FP64

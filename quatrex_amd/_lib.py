@@ -51,6 +51,8 @@ EXPORTED_SYMBOLS = (
     "rgf_graph_capture_solve",
     "rgf_graph_launch",
     "rgf_graph_destroy",
+    "rgf_densify_z",
+    "rgf_sparsify_z",
     "rgf_version",
 )
 
@@ -82,6 +84,15 @@ class MixedOpts_t(ctypes.Structure):
         ("fallback_flags", c_void_p),
         ("residuals", c_void_p),
     ]
+
+
+class BlockMapPart_t(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("src", ctypes.c_void_p), ("dst", ctypes.c_void_p),
+                ("op", ctypes.c_void_p)]
+
+
+class BlockMap_t(ctypes.Structure):
+    _fields_ = [("part", BlockMapPart_t * 3)]
 
 
 class Profile_t(ctypes.Structure):
@@ -133,6 +144,10 @@ def _declare(lib: ctypes.CDLL) -> None:
                                             P(Bt_t), P(Bt_t), P(Bt_t), P(c_void_p)]
     lib.rgf_graph_launch.argtypes = [c_void_p]
     lib.rgf_graph_destroy.argtypes = [c_void_p]
+    lib.rgf_densify_z.argtypes = [c_void_p, P(Geometry_t), c_int64, c_void_p, c_int64, P(BlockMap_t),
+                                  P(Bt_t)]
+    lib.rgf_sparsify_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t), P(BlockMap_t),
+                                   c_void_p, c_int64]
     lib.rgf_version.argtypes = []
     lib.rgf_version.restype = c_char_p
     for name in EXPORTED_SYMBOLS:

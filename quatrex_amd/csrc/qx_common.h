@@ -184,4 +184,15 @@ hipError_t launch_z2c(const double2* src, int64_t sSrc, float2* dst, int64_t sDs
 hipError_t launch_c2z(const float2* src, int64_t sSrc, double2* dst, int64_t sDst, int64_t count,
                       int nE, hipStream_t stream);
 
+// Block-sparse <-> packed tridiagonal (blocks.hip).  Entry k of a map moves
+// data[e][src[k]] to out[e][dst[k]] (densify) or back (sparsify, stored
+// entries only); op flags in blocks.hip.
+hipError_t launch_fill_zero(double2* p, int64_t count, int64_t stride, int nE, hipStream_t s);
+hipError_t launch_densify(const double2* data, int64_t sData, int64_t n, const int64_t* src,
+                          const int64_t* dst, const uint8_t* op, double2* out, int64_t sOut,
+                          int nE, hipStream_t s);
+hipError_t launch_sparsify(const double2* in, int64_t sIn, int64_t n, const int64_t* src,
+                           const int64_t* dst, const uint8_t* op, double2* data, int64_t sData,
+                           int nE, hipStream_t s);
+
 }  // namespace qx

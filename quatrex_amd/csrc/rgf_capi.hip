@@ -1412,6 +1412,67 @@ int rgf_graph_destroy(rgf_graph* graph) {
   return RGF_OK;
 }
 
+int rgf_densify_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies, const void* data,
+                  int64_t data_stride, const rgf_block_map* map, rgf_bt* out) {
+  if (!ctx) return RGF_EINVAL;
+  GUARD_BEGIN
+  qx::Geo g;
+  int rc = g.init(geom, &ctx->last_error);
+  if (rc) return rc;
+  if (n_energies < 0 || n_energies > 65535) return ctx->fail(RGF_EINVAL, "densify: 0 <= n_energies <= 65535");
+  if (n_energies == 0) return RGF_OK;
+  if (!data || !map || !out || data_stride < 0) return ctx->fail(RGF_EINVAL, "densify: data, map and out required");
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  double2* parts[3] = {static_cast<double2*>(out->diag), static_cast<double2*>(out->upper),
+                       static_cast<double2*>(out->lower)};
+  const int64_t size[3] = {g.Dsz, g.Usz, g.Usz};
+  for (int p = 0; p < 3; ++p) {
+    const rgf_block_map_part& m = map->part[p];
+    if (m.n > 0 && (!parts[p] || !m.src || !m.dst || !m.op))
+      return ctx->fail(RGF_EINVAL, "densify: map part without output or index arrays");
+  }
+  for (int p = 0; p < 3; ++p) {
+    if (!parts[p]) continue;
+    rc = ctx->hip(launch_fill_zero(parts[p], size[p], size[p], (int)n_energies, ctx->stream), "densify zero");
+    if (rc) return rc;
+    const rgf_block_map_part& m = map->part[p];
+    rc = ctx->hip(launch_densify(static_cast<const double2*>(data), data_stride, m.n, m.src, m.dst, m.op,
+                                 parts[p], size[p], (int)n_energies, ctx->stream),
+                  "densify");
+    if (rc) return rc;
+  }
+  return RGF_OK;
+  GUARD_END
+}
+
+int rgf_sparsify_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                   const rgf_bt_const* in, const rgf_block_map* map, void* data,
+                   int64_t data_stride) {
+  if (!ctx) return RGF_EINVAL;
+  GUARD_BEGIN
+  qx::Geo g;
+  int rc = g.init(geom, &ctx->last_error);
+  if (rc) return rc;
+  if (n_energies < 0 || n_energies > 65535) return ctx->fail(RGF_EINVAL, "sparsify: 0 <= n_energies <= 65535");
+  if (n_energies == 0) return RGF_OK;
+  if (!data || !map || !in || data_stride < 0) return ctx->fail(RGF_EINVAL, "sparsify: data, map and in required");
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  const double2* parts[3] = {static_cast<const double2*>(in->diag), static_cast<const double2*>(in->upper),
+                             static_cast<const double2*>(in->lower)};
+  const int64_t size[3] = {g.Dsz, g.Usz, g.Usz};
+  for (int p = 0; p < 3; ++p) {
+    const rgf_block_map_part& m = map->part[p];
+    if (m.n <= 0 || !parts[p]) continue;  // parts not given are left untouched in data
+    if (!m.src || !m.dst || !m.op) return ctx->fail(RGF_EINVAL, "sparsify: map part without index arrays");
+    rc = ctx->hip(launch_sparsify(parts[p], size[p], m.n, m.src, m.dst, m.op,
+                                  static_cast<double2*>(data), data_stride, (int)n_energies, ctx->stream),
+                  "sparsify");
+    if (rc) return rc;
+  }
+  return RGF_OK;
+  GUARD_END
+}
+
 int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset) {
   if (!ctx || !out) return RGF_EINVAL;
   int rc = ctx->prof_collect();
