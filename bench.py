@@ -58,19 +58,19 @@ def parse():
                     help="energies of the torch (rocBLAS/rocSOLVER) reference-algorithm leg (0 = skip)")
     ap.add_argument("--plain-sample", type=int, default=8,
                     help="energies for the one-energy-at-a-time unfused FP64 reference point")
-    ap.add_argument("--serving-requests", type=int, default=0,
-                    help="requests of the serving leg (a prefix of the Poisson stream; 0 = skip). "
-                         "Off by default: its process ended with SIGSEGV in 2 of its last 3 runs "
-                         "and the cause is not found yet")
+    ap.add_argument("--serving-requests", type=int, default=1000,
+                    help="requests of the serving leg (a prefix of the Poisson stream; 0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=2000.0, help="offered load, requests/s")
     ap.add_argument("--serving-budget-gib", type=float, default=96.0)
     ap.add_argument("--eta-sample", type=int, default=32,
                     help="energies per eta of the 48-block mixed-precision error leg (0 = skip)")
+    ap.add_argument("--dsdb-energies", type=int, default=64,
+                    help="energies of the block-sparse densify/sparsify leg (0 = skip)")
     ap.add_argument("--uneven-energies", type=int, default=2048,
                     help="energies of the uneven-batching leg over 4 geometries (0 = skip)")
     ap.add_argument("--uneven-capacity", type=int, default=64)
     ap.add_argument("--uneven-budget-gib", type=float, default=200.0)
-    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving"), default=None,
+    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving", "dsdb"), default=None,
                     help=argparse.SUPPRESS)
     ap.add_argument("--main-eps", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--leg-timeout", type=float, default=300.0)
@@ -331,6 +331,86 @@ def uneven_leg(args, dev) -> dict:
     }
 
 
+def _tridiagonal_coo(g, energies_vals):
+    """Block-sorted (rows, cols) of the full tridiagonal pattern and the matching
+    data of packed (diag, upper, lower) per energy."""
+    bs = list(g.block_sizes)
+    off = np.concatenate(([0], np.cumsum(bs)))
+    rows, cols, pieces = [], [], []
+    d, u, l = energies_vals
+    for i in range(g.n_blocks):
+        for j, part in ((i - 1, "l"), (i, "d"), (i + 1, "u")):
+            if j < 0 or j >= g.n_blocks:
+                continue
+            r, c = np.meshgrid(np.arange(bs[i]), np.arange(bs[j]), indexing="ij")
+            rows.append((off[i] + r).ravel())
+            cols.append((off[j] + c).ravel())
+            if part == "d":
+                pieces.append(g.diag(d, i).reshape(d.shape[0], -1))
+            elif part == "u":
+                pieces.append(g.upper(u, i).reshape(u.shape[0], -1))
+            else:
+                pieces.append(g.lower(l, j).reshape(l.shape[0], -1))
+    return np.concatenate(rows), np.concatenate(cols), pieces
+
+
+def dsdb_leg(args, dev) -> dict:
+    """Block-sparse stack <-> packed tridiagonal blocks (rgf_densify_z /
+    rgf_sparsify_z) at the bench geometry with the full tridiagonal pattern."""
+    import torch
+
+    from quatrex_amd import dsdb as D
+    from quatrex_amd import solve as S
+    from quatrex_amd import workloads as W
+    from quatrex_amd.layout import Geometry
+
+    g = Geometry.uniform(args.n_blocks, args.block_size)
+    E = args.dsdb_energies
+    model = W.make_device(g, W.DEVICE_SEED_BASE + 2)
+    A = W.system_matrix_torch(model, W.draw_energies(E), args.eta, dev)
+    rows, cols, pieces = _tridiagonal_coo(g, A)
+    data = torch.cat(pieces, dim=1).contiguous()
+    del pieces
+    a = D.DSDBCOO(data, rows, cols, g.block_sizes, (E,))
+    ctx = S.default_context(dev)
+    out = {"workload": f"{g.n_blocks} blocks x {args.block_size}, full tridiagonal pattern "
+                       f"({a.nnz} stored entries per energy), {E} energies, complex128"}
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            t0.record()
+            fn()
+            t1.record()
+            t1.synchronize()
+            best = min(best, t0.elapsed_time(t1))
+        return best * 1e-3
+
+    parts = {}
+
+    def dens():
+        parts["p"], _ = D.densify_batch(a, slice(None), g, "dul", dev, ctx)
+
+    t = timed(dens)
+    moved = 2 * 16 * a.nnz * E  # read every stored entry, write every dense element (same count)
+    out["densify"] = {"ms": round(t * 1e3, 3), "achieved_gbs": round(moved / t / 1e9, 1),
+                      "frac_hbm_peak": round(moved / t / 1e9 / HBM_PEAK_GBS, 4)}
+    x = D.DSDBCOO.zeros_like(a)
+    bufs = parts["p"]
+
+    def spar():
+        D.sparsify_batch(x, slice(None), g, bufs, "dul", dev, ctx)
+
+    t = timed(spar)
+    out["sparsify"] = {"ms": round(t * 1e3, 3), "achieved_gbs": round(moved / t / 1e9, 1),
+                       "frac_hbm_peak": round(moved / t / 1e9 / HBM_PEAK_GBS, 4)}
+    out["round_trip_exact"] = bool(torch.equal(x.data, a.data))
+    return out
+
+
 def serving_leg(args, dev) -> dict:
     """Continuous batching over 6 resident geometries, Poisson arrivals, adaptive precision."""
     from quatrex_amd import serving as SV
@@ -374,7 +454,7 @@ def serving_leg(args, dev) -> dict:
     }
 
 
-LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_sample",
+LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_sample", "dsdb_energies",
             "uneven_energies", "uneven_capacity", "uneven_budget_gib", "serving_requests",
             "serving_rate", "serving_budget_gib")
 
@@ -427,6 +507,8 @@ def leg_main(args) -> None:
         res = eta_leg(args, seed, dev)
     elif args.only_leg == "uneven":
         res = uneven_leg(args, dev)
+    elif args.only_leg == "dsdb":
+        res = dsdb_leg(args, dev)
     else:
         res = serving_leg(args, dev)
     print(json.dumps(res), flush=True)
@@ -659,13 +741,12 @@ def main():
         legs.append(("torch_fp64", "torch"))
     if args.eta_sample > 0 and not args.no_mixed:
         legs.append(("mixed_error_vs_eta", "eta"))
+    if args.dsdb_energies > 0:
+        legs.append(("block_sparse_io", "dsdb"))
     if args.uneven_energies > 0:
         legs.append(("uneven_batching", "uneven"))
     if args.serving_requests > 0:
         legs.append(("serving", "serving"))
-    else:
-        out["serving"] = {"skipped": "off by default: the serving leg's process ended with SIGSEGV "
-                                     "in 2 of its last 3 runs; cause not found yet"}
     for key, leg in legs:
         out[key] = run_leg(leg, args, eps)
 

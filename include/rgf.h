@@ -215,8 +215,9 @@ int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset);
  * 1 = mirror image of a stored entry under symmetric storage (densify only,
  * value op(v)); 2 = stored diagonal element of a symmetric diagonal block
  * (densifies to (v + op(v)) / 2); op(v) = s * conj(v), or s * v with flag 8,
- * s = -1 with flag 4.  rgf_densify_z zeroes every non-NULL output part first
- * (blocks absent from the pattern read as zeros); rgf_sparsify_z writes the
+ * s = -1 with flag 4.  rgf_densify_z zeroes every non-NULL output part whose
+ * map does not cover it (blocks absent from the pattern read as zeros);
+ * rgf_sparsify_z writes the
  * stored entries only (values outside the pattern are dropped, as in the
  * reference). */
 typedef struct rgf_block_map_part {
@@ -224,6 +225,7 @@ typedef struct rgf_block_map_part {
   const int64_t* src;
   const int64_t* dst;
   const uint8_t* op;
+  int32_t covers_part; /* 1: the entries write every element of the part (no zero fill needed) */
 } rgf_block_map_part;
 typedef struct rgf_block_map {
   rgf_block_map_part part[3];
@@ -238,7 +240,9 @@ int rgf_sparsify_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
  * rgf_graph_capture_solve records every launch of one sweep over n_energies
  * energies (precision 0: FP64; 1: FP32-compute products with FP64
  * inversions and `refine` Newton steps, without the a-posteriori check) into
- * a graph that owns its workspace and geometry tables; rgf_graph_launch
+ * a graph that owns its geometry tables and its workspace - or, for FP64,
+ * uses the context's caller-provided workspace (rgf_set_workspace) when that
+ * is large enough, which must then outlive the graph; rgf_graph_launch
  * replays it on the context's stream with whatever data the buffers hold at
  * that time.  The current is not supported in graphs.  The context must stay
  * alive while the graph is launched; rgf_graph_destroy waits for the last

@@ -88,7 +88,7 @@ class MixedOpts_t(ctypes.Structure):
 
 class BlockMapPart_t(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("src", ctypes.c_void_p), ("dst", ctypes.c_void_p),
-                ("op", ctypes.c_void_p)]
+                ("op", ctypes.c_void_p), ("covers_part", ctypes.c_int32)]
 
 
 class BlockMap_t(ctypes.Structure):

@@ -32,46 +32,54 @@ fill_zero_kernel(double2* p, int64_t count, int64_t stride) {
     q[i] = cmk(0.0, 0.0);
 }
 
+// Energies per thread: a map entry is loaded once and applied to kEPT
+// energies (the map is shared by all energies of the stack).
+constexpr int kEPT = 8;
+
 __global__ void __launch_bounds__(256)
 densify_kernel(const double2* __restrict__ data, int64_t sData, int64_t n,
                const int64_t* __restrict__ src, const int64_t* __restrict__ dst,
-               const uint8_t* __restrict__ op, double2* __restrict__ out, int64_t sOut) {
-  const int e = blockIdx.y;
-  const double2* d = data + (int64_t)e * sData;
-  double2* o = out + (int64_t)e * sOut;
+               const uint8_t* __restrict__ op, double2* __restrict__ out, int64_t sOut, int nE) {
+  const int e0 = blockIdx.y * kEPT;
+  const int ne = (nE - e0 < kEPT) ? (nE - e0) : kEPT;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x) {
-    const double2 v = d[src[k]];
+    const int64_t s = src[k], d = dst[k];
     const uint8_t f = op[k];
-    double2 w = v;
-    if (f & kMirror) {
-      w = sym_op(v, f);
-    } else if (f & kDiagSym) {
-      const double2 t = sym_op(v, f);
-      w = cmk(0.5 * (v.x + t.x), 0.5 * (v.y + t.y));
+    for (int u = 0; u < ne; ++u) {
+      const double2 v = data[(int64_t)(e0 + u) * sData + s];
+      double2 w = v;
+      if (f & kMirror) {
+        w = sym_op(v, f);
+      } else if (f & kDiagSym) {
+        const double2 t = sym_op(v, f);
+        w = cmk(0.5 * (v.x + t.x), 0.5 * (v.y + t.y));
+      }
+      out[(int64_t)(e0 + u) * sOut + d] = w;
     }
-    o[dst[k]] = w;
   }
 }
 
 __global__ void __launch_bounds__(256)
 sparsify_kernel(const double2* __restrict__ in, int64_t sIn, int64_t n,
                 const int64_t* __restrict__ src, const int64_t* __restrict__ dst,
-                const uint8_t* __restrict__ op, double2* __restrict__ data, int64_t sData) {
-  const int e = blockIdx.y;
-  const double2* p = in + (int64_t)e * sIn;
-  double2* d = data + (int64_t)e * sData;
+                const uint8_t* __restrict__ op, double2* __restrict__ data, int64_t sData, int nE) {
+  const int e0 = blockIdx.y * kEPT;
+  const int ne = (nE - e0 < kEPT) ? (nE - e0) : kEPT;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x) {
-    if (!(op[k] & kMirror)) d[src[k]] = p[dst[k]];
+    if (op[k] & kMirror) continue;
+    const int64_t s = src[k], d = dst[k];
+    for (int u = 0; u < ne; ++u)
+      data[(int64_t)(e0 + u) * sData + s] = in[(int64_t)(e0 + u) * sIn + d];
   }
 }
 
-dim3 grid_for(int64_t n, int nE) {
+dim3 grid_for(int64_t n, int ny) {
   int64_t gx = (n + 255) / 256;
   if (gx > 4096) gx = 4096;
   if (gx < 1) gx = 1;
-  return dim3((unsigned)gx, (unsigned)nE);
+  return dim3((unsigned)gx, (unsigned)ny);
 }
 
 }  // namespace
@@ -86,8 +94,8 @@ hipError_t launch_densify(const double2* data, int64_t sData, int64_t n, const i
                           const int64_t* dst, const uint8_t* op, double2* out, int64_t sOut,
                           int nE, hipStream_t s) {
   if (n <= 0 || nE <= 0) return hipSuccess;
-  hipLaunchKernelGGL(densify_kernel, grid_for(n, nE), dim3(256), 0, s, data, sData, n, src, dst, op,
-                     out, sOut);
+  hipLaunchKernelGGL(densify_kernel, grid_for(n, (nE + kEPT - 1) / kEPT), dim3(256), 0, s, data, sData,
+                     n, src, dst, op, out, sOut, nE);
   return hipGetLastError();
 }
 
@@ -95,8 +103,8 @@ hipError_t launch_sparsify(const double2* in, int64_t sIn, int64_t n, const int6
                            const int64_t* dst, const uint8_t* op, double2* data, int64_t sData,
                            int nE, hipStream_t s) {
   if (n <= 0 || nE <= 0) return hipSuccess;
-  hipLaunchKernelGGL(sparsify_kernel, grid_for(n, nE), dim3(256), 0, s, in, sIn, n, src, dst, op,
-                     data, sData);
+  hipLaunchKernelGGL(sparsify_kernel, grid_for(n, (nE + kEPT - 1) / kEPT), dim3(256), 0, s, in, sIn,
+                     n, src, dst, op, data, sData, nE);
   return hipGetLastError();
 }
 

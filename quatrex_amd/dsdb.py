@@ -110,7 +110,8 @@ class _DSDB:
     def tridiagonal_map(self, device) -> SB.DeviceMap:
         key = ("tri", str(device))
         if key not in self._maps:
-            self._maps[key] = SB.DeviceMap(SB.map_of(self, self.geometry), device)
+            self._maps[key] = SB.DeviceMap(SB.map_of(self, self.geometry), device,
+                                           _part_sizes(self.geometry))
         return self._maps[key]
 
     def _coordinates(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -137,7 +138,7 @@ class _DSDB:
                 geo = Geometry((int(bs[si]), int(bs[sj])))
                 part = "l" if (si, sj) != (i, j) else "u"
                 hm = SB.map_from_coordinates(geo, r, c + int(bs[si]), p, self.symmetry, self.symmetry_op)
-            self._maps[key] = (SB.DeviceMap(hm, device), geo, part)
+            self._maps[key] = (SB.DeviceMap(hm, device, _part_sizes(geo)), geo, part)
         return self._maps[key]
 
     def _ctx(self):
@@ -297,6 +298,85 @@ class DSDBCSR(_DSDB):
         vals = torch.as_tensor(coo.data[order].astype(np.complex128), device=device)
         data = vals.expand(*global_stack_shape, vals.numel()).contiguous()
         return cls(data, cols, rowptr_map, block_sizes, global_stack_shape, symmetry, symmetry_op)
+
+
+def _part_sizes(geo: Geometry) -> tuple:
+    return (geo.diag_size, geo.offdiag_size, geo.offdiag_size)
+
+
+def device_map(m, geo: Geometry, device) -> SB.DeviceMap:
+    """The tridiagonal gather map of ``m`` on ``device`` (cached on the object)."""
+    if hasattr(m, "tridiagonal_map"):
+        return m.tridiagonal_map(device)
+    cache = m.__dict__.setdefault("_qx_tridiagonal_maps", {})
+    key = str(device)
+    if key not in cache:
+        cache[key] = SB.DeviceMap(SB.map_of(m, geo), device, _part_sizes(geo))
+    return cache[key]
+
+
+def _batch_data(m, sl, device):
+    """The batch's data as a contiguous ``(E, nnz)`` complex128 device tensor."""
+    torch = _torch()
+    x = m.data[sl]
+    lead = tuple(x.shape[:-1])
+    nnz = int(x.shape[-1])
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=device, dtype=torch.complex128)
+    else:
+        host = np.ascontiguousarray(SB._host(x), dtype=np.complex128)
+        t = torch.from_numpy(host)
+        t = (t.pin_memory() if torch.cuda.is_available() else t).to(device, non_blocking=True)
+    return t.reshape(-1, nnz).contiguous(), lead
+
+
+def densify_batch(m, sl, geo: Geometry, parts: str, device, ctx):
+    """Packed ``(diag, upper, lower)`` of a batch of a block-sparse stack: one
+    densify launch per part for all energies (rgf_densify_z)."""
+    torch = _torch()
+    data, lead = _batch_data(m, sl, device)
+    E = int(data.shape[0])
+    sizes = {"d": geo.diag_size, "u": geo.offdiag_size, "l": geo.offdiag_size}
+    out = {k: torch.empty((E, sizes[k]), dtype=torch.complex128, device=device) for k in parts}
+    dmap = device_map(m, geo, device)
+    bt = _lib.Bt_t(*(out[k].data_ptr() if k in out else None for k in "dul"))
+    g, keep = _lib.geometry_struct(geo.block_sizes)
+    rc = _lib.load().rgf_densify_z(ctx.handle, ctypes.byref(g), E, ctypes.c_void_p(data.data_ptr()),
+                                   int(data.shape[1]), ctypes.byref(dmap.restricted(parts)),
+                                   ctypes.byref(bt))
+    _lib.check(rc, ctx.handle)
+    del keep
+    return tuple(out.get(k) for k in "dul"), lead
+
+
+def sparsify_batch(m, sl, geo: Geometry, bufs, parts: str, device, ctx) -> None:
+    """Write packed results into the stored entries of a block-sparse stack
+    batch (rgf_sparsify_z); entries outside the written blocks keep their values."""
+    torch = _torch()
+    view = m.data[sl]
+    direct = (isinstance(view, torch.Tensor) and view.device == torch.device(device)
+              and view.dtype == torch.complex128 and view.is_contiguous())
+    if direct:
+        data = view.reshape(-1, view.shape[-1])
+    else:
+        data, _ = _batch_data(m, sl, device)
+    E = int(data.shape[0])
+    dmap = device_map(m, geo, device)
+    use = "".join(k for k, t in zip("dul", bufs) if t is not None and k in parts)
+    bt = _lib.BtConst_t(*(t.data_ptr() if (t is not None and k in use) else None for k, t in zip("dul", bufs)))
+    g, keep = _lib.geometry_struct(geo.block_sizes)
+    rc = _lib.load().rgf_sparsify_z(ctx.handle, ctypes.byref(g), E, ctypes.byref(bt),
+                                    ctypes.byref(dmap.restricted(use)), ctypes.c_void_p(data.data_ptr()),
+                                    int(data.shape[1]))
+    _lib.check(rc, ctx.handle)
+    del keep
+    if not direct:
+        res = data.reshape(view.shape)
+        if isinstance(view, torch.Tensor):
+            view.copy_(res.to(view.device, view.dtype))
+        else:
+            view[...] = res.cpu().numpy().astype(view.dtype, copy=False)
+
 
 
 class _Stack:

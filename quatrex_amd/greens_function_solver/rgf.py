@@ -26,13 +26,11 @@ blocks between the caller's containers and packed device buffers.
 
 from __future__ import annotations
 
-import ctypes
-
 import numpy as np
 
-from quatrex_amd import _lib
 from quatrex_amd import solve as _solve
-from quatrex_amd import sparse_blocks as _sb
+from quatrex_amd.dsdb import densify_batch as _gather_dsdb
+from quatrex_amd.dsdb import sparsify_batch as _scatter_dsdb
 from quatrex_amd.datastructures import BlockTridiagonal
 from quatrex_amd.greens_function_solver.solver import GFSolver, OBCBlocks
 from quatrex_amd.layout import Geometry
@@ -144,79 +142,6 @@ def _is_dsdb(m) -> bool:
     cols) layout: ``data[*stack, nnz]`` (dsdbcoo.py:23-120, dsdbcsr.py)."""
     return (hasattr(m, "data") and hasattr(m, "block_sizes") and hasattr(m, "cols")
             and (hasattr(m, "rows") or isinstance(getattr(m, "rowptr_map", None), dict)))
-
-
-def _dsdb_map(m, geo: Geometry, device):
-    if hasattr(m, "tridiagonal_map"):
-        return m.tridiagonal_map(device)
-    cache = m.__dict__.setdefault("_qx_tridiagonal_maps", {})
-    key = str(device)
-    if key not in cache:
-        cache[key] = _sb.DeviceMap(_sb.map_of(m, geo), device)
-    return cache[key]
-
-
-def _dsdb_data(m, sl, device):
-    """The batch's data as a contiguous ``(E, nnz)`` complex128 device tensor."""
-    torch = _torch()
-    x = m.data[sl]
-    lead = tuple(x.shape[:-1])
-    nnz = int(x.shape[-1])
-    if isinstance(x, torch.Tensor):
-        t = x.to(device=device, dtype=torch.complex128)
-    else:
-        host = np.ascontiguousarray(_sb._host(x), dtype=np.complex128)
-        t = torch.from_numpy(host)
-        t = (t.pin_memory() if torch.cuda.is_available() else t).to(device, non_blocking=True)
-    return t.reshape(-1, nnz).contiguous(), lead
-
-
-def _gather_dsdb(m, sl, geo: Geometry, parts: str, device, ctx):
-    """Packed ``(diag, upper, lower)`` of a batch of a block-sparse stack: one
-    densify launch per part for all energies (rgf_densify_z)."""
-    torch = _torch()
-    data, lead = _dsdb_data(m, sl, device)
-    E = int(data.shape[0])
-    sizes = {"d": geo.diag_size, "u": geo.offdiag_size, "l": geo.offdiag_size}
-    out = {k: torch.empty((E, sizes[k]), dtype=torch.complex128, device=device) for k in parts}
-    dmap = _dsdb_map(m, geo, device)
-    bt = _lib.Bt_t(*(out[k].data_ptr() if k in out else None for k in "dul"))
-    g, keep = _lib.geometry_struct(geo.block_sizes)
-    rc = _lib.load().rgf_densify_z(ctx.handle, ctypes.byref(g), E, ctypes.c_void_p(data.data_ptr()),
-                                   int(data.shape[1]), ctypes.byref(dmap.restricted(parts)),
-                                   ctypes.byref(bt))
-    _lib.check(rc, ctx.handle)
-    del keep
-    return tuple(out.get(k) for k in "dul"), lead
-
-
-def _scatter_dsdb(m, sl, geo: Geometry, bufs, parts: str, device, ctx) -> None:
-    """Write packed results into the stored entries of a block-sparse stack
-    batch (rgf_sparsify_z); entries outside the written blocks keep their values."""
-    torch = _torch()
-    view = m.data[sl]
-    direct = (isinstance(view, torch.Tensor) and view.device == torch.device(device)
-              and view.dtype == torch.complex128 and view.is_contiguous())
-    if direct:
-        data = view.reshape(-1, view.shape[-1])
-    else:
-        data, _ = _dsdb_data(m, sl, device)
-    E = int(data.shape[0])
-    dmap = _dsdb_map(m, geo, device)
-    use = "".join(k for k, t in zip("dul", bufs) if t is not None and k in parts)
-    bt = _lib.BtConst_t(*(t.data_ptr() if (t is not None and k in use) else None for k, t in zip("dul", bufs)))
-    g, keep = _lib.geometry_struct(geo.block_sizes)
-    rc = _lib.load().rgf_sparsify_z(ctx.handle, ctypes.byref(g), E, ctypes.byref(bt),
-                                    ctypes.byref(dmap.restricted(use)), ctypes.c_void_p(data.data_ptr()),
-                                    int(data.shape[1]))
-    _lib.check(rc, ctx.handle)
-    del keep
-    if not direct:
-        res = data.reshape(view.shape)
-        if isinstance(view, torch.Tensor):
-            view.copy_(res.to(view.device, view.dtype))
-        else:
-            view[...] = res.cpu().numpy().astype(view.dtype, copy=False)
 
 
 def _zeros_current(like_container, shape):

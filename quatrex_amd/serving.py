@@ -17,9 +17,11 @@ geometries - and several slots of one geometry - run concurrently on the
 device.  A request's latency is the time from its arrival to the end of the
 solve that produced it (results resident in HBM).
 
-FP64 batches replay a hipGraph of the whole sweep captured once per batch
-size for the slot's fixed buffers (A is re-assembled in place before each
-replay), which removes the per-launch host cost of small geometries.
+FP64 batches replay a hipGraph of the whole sweep, captured for every batch
+size of a slot when the server starts (serially, before any batch runs; the
+graphs share the slot's workspace) and replayed on the slot's buffers (A is
+re-assembled in place before each replay), which removes the per-launch host
+cost of small geometries.
 
 Precision: ``mixed`` runs the complex64-product sweep with the FP64 check and
 FP64 fallback above ``fallback_tol``; ``adaptive`` additionally sends a
@@ -160,11 +162,8 @@ class Slot:
                       ctx=self.ctx)
         SL, SG = view(self.SL), view(self.SG)
         if precision == "fp64":
-            if self.server.use_graphs:
-                graph = self.graphs.get(n)
-                if graph is None:
-                    graph = S.SolveGraph(g, A, SL, SG, **common)
-                    self.graphs[n] = graph
+            graph = self.graphs.get(n)
+            if graph is not None:
                 graph.launch()
             else:
                 S.selected_solve(g, A, SL, SG, **common)
@@ -175,6 +174,18 @@ class Slot:
         S.selected_solve(g, A, SL, SG, precision="mixed", mixed=opts, **common)
         self.ctx.status()
         return self.flags[:n].cpu().numpy().astype(bool)
+
+    def prepare_graphs(self) -> None:
+        """Capture the FP64 sweep for every batch size 1..capacity (server thread,
+        before any batch runs): graphs are never captured while other slots
+        launch work, and they share the slot's workspace."""
+        g = self.geometry
+        view = lambda T, n: tuple(t[:n] if t is not None else None for t in T)  # noqa: E731
+        for n in range(1, self.capacity + 1):
+            self.graphs[n] = S.SolveGraph(g, view(self.A, n), view(self.SL, n), view(self.SG, n),
+                                          g_lesser=view(self.GL, n), g_greater=view(self.GG, n),
+                                          g_retarded=view(self.GR, n), ctx=self.ctx)
+        self.stream.synchronize()
 
     def outputs(self, k: int):
         """Device views of energy ``k`` of the last batch: (G^R, G^<, G^>) packed."""
@@ -258,6 +269,10 @@ class Server:
         self.tiles: dict = {}
         if autotune:
             self._autotune(geometries, capacity)
+        if use_graphs and precision != "mixed":
+            # serially, here: the tile choice is final and no batch is running
+            for slot in self.slots:
+                slot.prepare_graphs()
         self.recent = [deque(maxlen=4) for _ in geometries]
         self.batches_since_retry = [0] * len(geometries)
         self.retry_gap = [retry_every] * len(geometries)

@@ -152,7 +152,7 @@ def _host(x):
 class DeviceMap:
     """A map resident on one device, with the ctypes struct the library takes."""
 
-    def __init__(self, host_map: dict, device):
+    def __init__(self, host_map: dict, device, part_sizes=None):
         import torch
 
         from quatrex_amd import _lib
@@ -161,12 +161,15 @@ class DeviceMap:
         self.struct = _lib.BlockMap_t()
         for k, key in enumerate(PARTS):
             src, dst, op = host_map[key]
+            # every destination appears at most once, so a full count is full coverage
+            covers = part_sizes is not None and dst.size == part_sizes[k]
             t = (torch.as_tensor(src, device=device), torch.as_tensor(dst, device=device),
                  torch.as_tensor(op, device=device))
             self.tensors[key] = t
             part = self.struct.part[k]
             part.n = int(src.size)
             part.src, part.dst, part.op = (x.data_ptr() if x.numel() else None for x in t)
+            part.covers_part = int(covers)
 
     def restricted(self, parts: str):
         """The ctypes struct with only ``parts`` (others empty)."""
