@@ -70,7 +70,9 @@ def parse():
                     help="energies of the uneven-batching leg over 4 geometries (0 = skip)")
     ap.add_argument("--uneven-capacity", type=int, default=64)
     ap.add_argument("--uneven-budget-gib", type=float, default=200.0)
-    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving", "dsdb"), default=None,
+    ap.add_argument("--obc-energies", type=int, default=512,
+                    help="energies of the open-boundary (Sancho-Rubio) leg (0 = skip)")
+    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving", "dsdb", "obc"), default=None,
                     help=argparse.SUPPRESS)
     ap.add_argument("--main-eps", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--leg-timeout", type=float, default=300.0)
@@ -411,6 +413,92 @@ def dsdb_leg(args, dev) -> dict:
     return out
 
 
+def obc_leg(args, dev) -> dict:
+    """Open boundary conditions of both contacts of the bench device for one
+    energy batch: boundary system, Sancho-Rubio surface Green's function,
+    refinement step with residuals, retarded/lesser/greater self-energies."""
+    import torch
+
+    from quatrex_amd import solve as S
+    from quatrex_amd import workloads as W
+    from quatrex_amd.boundary_conditions import OBCSystem, SanchoRubio, compute_contact_obc
+    from quatrex_amd.layout import Geometry
+
+    g = Geometry.uniform(args.n_blocks, args.block_size)
+    E, b, n = args.obc_energies, args.block_size, args.n_blocks - 1
+    model = W.make_device(g, W.DEVICE_SEED_BASE + 2)
+    z = torch.as_tensor(W.draw_energies(E) + 1j * args.eta, device=dev)
+    eye = torch.eye(b, dtype=torch.complex128, device=dev)
+
+    def blk(packed, view, i):
+        return torch.as_tensor(view(packed, i).copy(), device=dev)
+
+    # boundary blocks of A = (E + i eta) I - H (no boundary term)
+    def diag(i):
+        return (z[:, None, None] * eye - blk(model.h_diag, g.diag, i)).contiguous()
+
+    def coupling(view, i):
+        return (-blk(view[0], view[1], i)).expand(E, b, b).contiguous()
+
+    up, lo = (model.h_upper, g.upper), (model.h_lower, g.lower)
+    left = (coupling(lo, 0), diag(0), coupling(up, 0))
+    right = (coupling(up, n - 1), diag(n), coupling(lo, n - 1))
+    fl = torch.full((E,), 0.9, dtype=torch.float64, device=dev)
+    fr = torch.full((E,), 0.1, dtype=torch.float64, device=dev)
+    ctx = S.default_context(dev)
+    out = {"workload": f"both contacts of {args.n_blocks} blocks x {b}, {E} energies, eta={args.eta}, "
+                       "eta_obc=0, Sancho-Rubio(max_iterations=100, convergence_tol=1e-6)"}
+
+    def run(osys):
+        res = []
+        for name, (m10, m00, m01), f in (("left", left, fl), ("right", right, fr)):
+            res.append(compute_contact_obc(m10, m00, m01, f, 0.0, osys, contact=name))
+        return res
+
+    fresh = SanchoRubio(100, 1e-6)
+    run(OBCSystem(fresh, memoization_mode="off"))  # warm-up
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    t0 = time.perf_counter()
+    run(OBCSystem(fresh, memoization_mode="off"))
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    it = fresh.iterations
+    # algorithmic work: per decimation step 6 products and 1 inverse (8 b^3 each),
+    # plus the final inverse, the refinement step (2 products + 1 inverse) and
+    # the self-energy (2 products), per contact and energy
+    flops = 2 * E * 8.0 * b**3 * (7 * it + 1 + 3 + 2)
+    out["sancho_rubio"] = {
+        "ms": round(t * 1e3, 3), "energies_per_s": round(E / t, 1), "iterations_last_contact": it,
+        "delta_last_contact": float(f"{fresh.delta:.3e}"),
+        "achieved_tflops": round(flops / t / 1e12, 3),
+        "frac_fp64_peak": round(flops / t / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "inverse_ms_share": round(prof["inv_ms"] / max(prof["inv_ms"] + prof["gemm_ms"], 1e-9), 4),
+        "gemm_tflops": round(prof["gemm_flops"] / max(prof["gemm_ms"], 1e-9) / 1e9, 3),
+    }
+    # memoised: the second SCBA-like call reuses the cached surface function
+    memo = OBCSystem(SanchoRubio(100, 1e-6), memoization_mode="force-after-first")
+    run(memo)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = run(memo)
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    out["memoized"] = {"ms": round(tm * 1e3, 3), "speedup_vs_fresh": round(t / tm, 2)}
+    # size-independent self-check: the boundary self-energy of a converged
+    # surface function has Gamma = i(S - S^H) Hermitian, and lesser + greater
+    # = i (2f - 1) Gamma is anti-Hermitian
+    sig = res[0][0]
+    herm = float((torch.linalg.norm(res[0][1] + res[0][1].conj().transpose(-1, -2)) /
+                  torch.linalg.norm(res[0][1])).item())
+    out["lesser_antihermitian_rel"] = float(f"{herm:.3e}")
+    out["sigma_norm"] = float(f"{float(torch.linalg.norm(sig[0])):.6e}")
+    return out
+
+
 def serving_leg(args, dev) -> dict:
     """Continuous batching over 6 resident geometries, Poisson arrivals, adaptive precision."""
     from quatrex_amd import serving as SV
@@ -455,6 +543,7 @@ def serving_leg(args, dev) -> dict:
 
 
 LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_sample", "dsdb_energies",
+            "obc_energies",
             "uneven_energies", "uneven_capacity", "uneven_budget_gib", "serving_requests",
             "serving_rate", "serving_budget_gib")
 
@@ -509,6 +598,8 @@ def leg_main(args) -> None:
         res = uneven_leg(args, dev)
     elif args.only_leg == "dsdb":
         res = dsdb_leg(args, dev)
+    elif args.only_leg == "obc":
+        res = obc_leg(args, dev)
     else:
         res = serving_leg(args, dev)
     print(json.dumps(res), flush=True)
@@ -743,6 +834,8 @@ def main():
         legs.append(("mixed_error_vs_eta", "eta"))
     if args.dsdb_energies > 0:
         legs.append(("block_sparse_io", "dsdb"))
+    if args.obc_energies > 0:
+        legs.append(("open_boundary", "obc"))
     if args.uneven_energies > 0:
         legs.append(("uneven_batching", "uneven"))
     if args.serving_requests > 0:

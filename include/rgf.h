@@ -236,6 +236,52 @@ int rgf_sparsify_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
                    const rgf_bt_const* in, const rgf_block_map* map, void* data,
                    int64_t data_stride);
 
+/* Open boundary conditions.  All matrices are n_energies stacked b x b
+ * complex128 device matrices (row-major, energy stride b*b).
+ *
+ * rgf_sancho_rubio_z: surface Green's function by Sancho-Rubio decimation,
+ * replacing SanchoRubio.__call__ (src/qttools/boundary_conditions/obc/
+ * sancho_rubio.py:33-85) for a whole energy batch.  Per iteration:
+ * inverse = inv(eps); eps -= alpha inverse beta + beta inverse alpha;
+ * eps_s -= alpha inverse beta; alpha = alpha inverse alpha; beta = beta
+ * inverse beta, starting from eps = eps_s = a_ii, alpha = a_ji, beta = a_ij.
+ * The loop stops after the first iteration whose delta = max over energies of
+ * ||(|alpha| + |beta|)||_F / 2 is below convergence_tol, or after
+ * max_iterations; x_ii = inv(eps_s).  *iterations (host) receives the number
+ * of iterations run and *delta (host) the last delta (NaN if none ran): the
+ * batch did not converge unless *delta < convergence_tol (the reference then
+ * warns).  Synchronises the context stream once per iteration (the stopping
+ * rule is evaluated on the host, as in the reference).
+ *
+ * rgf_obc_fixed_point_z: one refinement step x_ref = inv(a_ii - a_ji x a_ij)
+ * with the residuals abs = ||x_ref - x||_F and rel = abs / ||x_ref||_F per
+ * energy (device arrays, both NULL to skip), replacing
+ * OBCSystem._fixed_point_step / _get_residuals (obc.py:102-127, 154-202).
+ *
+ * rgf_contact_obc_z: contact self-energies from the surface Green's function,
+ * replacing ElectronSolver._compute_contact_obc (src/quatrex/electron/
+ * solver.py:486-495): sigma = m_10 g_00 m_01, gamma = i (sigma - sigma^H),
+ * lesser = i f gamma, greater = i (f - 1) gamma with f = occupancies[e]
+ * (device, float64).  lesser and greater may be NULL.
+ *
+ * rgf_boundary_system_z: the boundary system handed to the OBC solver,
+ * a_ii = m_00 + i eta s_00, a_ij = m_01 + i eta s_01, a_ji = m_10 + i eta s_10
+ * (solver.py:468-483): with the overlap blocks s_* NULL, s_00 is the identity
+ * and s_01 = s_10 = 0.  a_ij / a_ji may be NULL (not written); outputs may
+ * alias their inputs. */
+int rgf_sancho_rubio_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, const void* a_ii,
+                       const void* a_ij, const void* a_ji, int32_t max_iterations,
+                       double convergence_tol, void* x_ii, int32_t* iterations, double* delta);
+int rgf_obc_fixed_point_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, const void* a_ii,
+                          const void* a_ij, const void* a_ji, const void* x, void* x_ref,
+                          double* abs_residuals, double* rel_residuals);
+int rgf_contact_obc_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, const void* m_10,
+                      const void* g_00, const void* m_01, const double* occupancies, void* sigma,
+                      void* lesser, void* greater);
+int rgf_boundary_system_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, double eta_obc,
+                          const void* m_00, const void* m_01, const void* m_10, const void* s_00,
+                          const void* s_01, const void* s_10, void* a_ii, void* a_ij, void* a_ji);
+
 /* HIP graph of a complete selected-solve sweep for fixed buffers.
  * rgf_graph_capture_solve records every launch of one sweep over n_energies
  * energies (precision 0: FP64; 1: FP32-compute products with FP64

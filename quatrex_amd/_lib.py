@@ -53,6 +53,10 @@ EXPORTED_SYMBOLS = (
     "rgf_graph_destroy",
     "rgf_densify_z",
     "rgf_sparsify_z",
+    "rgf_sancho_rubio_z",
+    "rgf_obc_fixed_point_z",
+    "rgf_contact_obc_z",
+    "rgf_boundary_system_z",
     "rgf_version",
 )
 
@@ -148,6 +152,14 @@ def _declare(lib: ctypes.CDLL) -> None:
                                   P(Bt_t)]
     lib.rgf_sparsify_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t), P(BlockMap_t),
                                    c_void_p, c_int64]
+    lib.rgf_sancho_rubio_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                       c_int32, ctypes.c_double, c_void_p, P(c_int32),
+                                       P(ctypes.c_double)]
+    lib.rgf_obc_fixed_point_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.rgf_contact_obc_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, c_void_p]
+    lib.rgf_boundary_system_z.argtypes = [c_void_p, c_int64, c_int64, ctypes.c_double] + [c_void_p] * 9
     lib.rgf_version.argtypes = []
     lib.rgf_version.restype = c_char_p
     for name in EXPORTED_SYMBOLS:

@@ -902,8 +902,6 @@ int64_t Ctx::chunk_for(const Geo& g, int64_t nE, bool solve, bool current, bool 
 // C ABI
 // ---------------------------------------------------------------------------
 
-struct rgf_ctx : public qx::Ctx {};
-
 // Graph capture and graph launches never overlap across host threads: a
 // launch in one thread while another thread captured has ended the process
 // with SIGSEGV inside hipGraphLaunch.  Captures hold this lock exclusively,
@@ -930,16 +928,6 @@ struct rgf_graph {
     if (tables) (void)hipFree(tables);
   }
 };
-
-#define GUARD_BEGIN try {
-#define GUARD_END                                              \
-  }                                                            \
-  catch (const std::bad_alloc&) {                              \
-    return ctx ? ctx->fail(RGF_ENOMEM, "host allocation failed") : RGF_ENOMEM; \
-  }                                                            \
-  catch (...) {                                                \
-    return ctx ? ctx->fail(RGF_EHIP, "internal error") : RGF_EHIP; \
-  }
 
 extern "C" {
 

@@ -1,6 +1,7 @@
 // Host-side driver types shared by the FP64 and mixed-precision sweeps.
 #pragma once
 
+#include <new>
 #include <string>
 #include <vector>
 
@@ -95,3 +96,17 @@ struct Ctx {
 };
 
 }  // namespace qx
+
+// The opaque handle of the C ABI.
+struct rgf_ctx : public qx::Ctx {};
+
+// C ABI bodies: no C++ exception crosses the interface.
+#define GUARD_BEGIN try {
+#define GUARD_END                                              \
+  }                                                            \
+  catch (const std::bad_alloc&) {                              \
+    return ctx ? ctx->fail(RGF_ENOMEM, "host allocation failed") : RGF_ENOMEM; \
+  }                                                            \
+  catch (...) {                                                \
+    return ctx ? ctx->fail(RGF_EHIP, "internal error") : RGF_EHIP; \
+  }
