@@ -119,6 +119,25 @@ def cpu_baseline(model, energies, eta, budget_s):
         t1 = run(1)
         n = int(max(1, min(64, budget_s // max(t1, 1e-3))))
         tn = run(n) if n > 1 else t1
+    # the same restatement on one core: one energy
+    single = None
+    if threadpool_limits:
+        with threadpool_limits(limits=1):
+            ts = run(1)
+        single = {"value": 1.0 / ts, "unit": "energy_points/s", "cores": 1,
+                  "sample": f"1 energy, 1 BLAS thread, {ts:.1f} s"}
+    cpu_model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu_model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    blas = None
+    if threadpool_limits:
+        from threadpoolctl import threadpool_info
+
+        blas = sorted({f"{i.get('internal_api')} {i.get('version')}" for i in threadpool_info()
+                       if i.get("user_api") == "blas"})
     return {
         "value": n / tn,
         "unit": "energy_points/s",
@@ -126,6 +145,9 @@ def cpu_baseline(model, energies, eta, budget_s):
         "kind": "port",
         "sample": f"{n} energies of the {g.n_blocks}x{g.block_sizes[0]} workload (numpy complex128 "
                   f"restatement of rgf.py, {cores} BLAS threads, {tn:.1f} s)",
+        "single_core": single,
+        "cpu_model": cpu_model,
+        "blas": blas,
     }
 
 

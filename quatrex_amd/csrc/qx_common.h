@@ -71,7 +71,7 @@ struct ZProblem {
 // Up to kMaxProblems independent problems, each over nE energies, in one launch.
 struct ZGroup {
   int32_t nprob, nE;
-  int32_t wg_base[kMaxProblems + 1];  // cumulative workgroup counts
+  int32_t wg_base[kMaxProblems + 1];  // cumulative tile counts within one energy
   int32_t tiles_n[kMaxProblems];
   int32_t tiles_mn[kMaxProblems];
   ZProblem p[kMaxProblems];

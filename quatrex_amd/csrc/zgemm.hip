@@ -150,14 +150,18 @@ zgemm_grouped_kernel(const ZGroup g) {
     return reinterpret_cast<LT(*)[TN + PAD]>(smem + 2 * L::kA + buf * L::kB);
   };
 
-  const int total = g.wg_base[g.nprob];
-  const int wid = xcd_remap(blockIdx.x, total);
+  // Energy-major order: logical id = e * per + (tile offset of the problem
+  // within the energy) + tile.  The XCD remap gives every XCD a contiguous
+  // range of energies with all their problems, so XCDs receive equal work even
+  // when the problems of a launch differ in cost (number of terms, addends).
+  const int per = g.wg_base[g.nprob];
+  const int wid = xcd_remap(blockIdx.x, per * g.nE);
+  const int e = wid / per;
+  const int r = wid - e * per;
   int p = 0;
-  while (p + 1 < g.nprob && wid >= g.wg_base[p + 1]) ++p;
+  while (p + 1 < g.nprob && r >= g.wg_base[p + 1]) ++p;
   const ZProblem& P = g.p[p];
-  const int local = wid - g.wg_base[p];
-  const int e = local / g.tiles_mn[p];
-  const int t = local % g.tiles_mn[p];
+  const int t = r - g.wg_base[p];
   const int m0 = (t / g.tiles_n[p]) * TM;
   const int n0 = (t % g.tiles_n[p]) * TN;
   const int m = P.m, n = P.n;
@@ -356,18 +360,20 @@ zgemm_grouped_kernel(const ZGroup g) {
 template <int TM, int TN, int kRole, bool kF32 = false>
 hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   ZGroup g = g0;
-  int base = 0;
+  int per = 0;  // workgroups per energy (all problems)
   for (int p = 0; p < g.nprob; ++p) {
     const int tm = (g.p[p].m + TM - 1) / TM, tn = (g.p[p].n + TN - 1) / TN;
     g.tiles_n[p] = tn;
     g.tiles_mn[p] = tm * tn;
-    g.wg_base[p] = base;
-    base += tm * tn * g.nE;
+    g.wg_base[p] = per;
+    per += tm * tn;
   }
-  g.wg_base[g.nprob] = base;
-  if (base == 0) return hipSuccess;
-  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole, kF32>), dim3(base), dim3(TileShape<TM, TN>::kThreads),
-                     0, stream, g);
+  g.wg_base[g.nprob] = per;
+  const int64_t total = (int64_t)per * g.nE;
+  if (total == 0) return hipSuccess;
+  if (total > 0x7fffffff) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole, kF32>), dim3((unsigned)total),
+                     dim3(TileShape<TM, TN>::kThreads), 0, stream, g);
   return hipGetLastError();
 }
 
