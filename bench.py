@@ -72,7 +72,10 @@ def parse():
     ap.add_argument("--uneven-budget-gib", type=float, default=200.0)
     ap.add_argument("--obc-energies", type=int, default=512,
                     help="energies of the open-boundary (Sancho-Rubio) leg (0 = skip)")
-    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving", "dsdb", "obc"), default=None,
+    ap.add_argument("--sections", type=int, default=4,
+                    help="block sections of the per-section (partitioned solve) leg (0 = skip)")
+    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving", "dsdb", "obc", "section"),
+                    default=None,
                     help=argparse.SUPPRESS)
     ap.add_argument("--main-eps", type=float, default=0.0, help=argparse.SUPPRESS)
     ap.add_argument("--leg-timeout", type=float, default=300.0)
@@ -521,6 +524,52 @@ def obc_leg(args, dev) -> dict:
     return out
 
 
+def section_leg(args, dev) -> dict:
+    """The per-section work of a block-partitioned (multi-GPU) solve at its
+    real shape: the FP64 sweep over n_blocks / sections blocks of the same
+    size and the same energies, as one GPU of the partition would run it."""
+    import torch
+
+    from quatrex_amd import solve as S
+    from quatrex_amd import workloads as W
+    from quatrex_amd.layout import Geometry
+
+    nb = args.n_blocks // args.sections
+    g = Geometry.uniform(nb, args.block_size)
+    E = args.energies
+    model = W.make_device(g, W.DEVICE_SEED_BASE + 2)
+    A = W.system_matrix_torch(model, W.draw_energies(E), args.eta, dev)
+    sld = torch.as_tensor(model.sl_diag, device=dev).expand(E, -1).contiguous()
+    sgd = torch.as_tensor(model.sg_diag, device=dev).expand(E, -1).contiguous()
+    slu = torch.zeros((E, g.offdiag_size), dtype=torch.complex128, device=dev)
+    sgu = torch.zeros_like(slu)
+    GR = S.packed_empty(g, E, dev)
+    GL = S.packed_empty(g, E, dev, parts=("d", "u"))
+    GG = S.packed_empty(g, E, dev, parts=("d", "u"))
+    ctx = S.default_context(dev)
+
+    def step():
+        S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
+                         ctx=ctx)
+
+    step()
+    ctx.status()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    ctx.status()
+    f = flops_per_energy(nb, args.block_size)
+    return {"workload": f"one of {args.sections} sections: {nb} blocks x {args.block_size}, {E} energies, "
+                        "FP64 selected solve",
+            "ms": round(t * 1e3, 3), "energies_per_s": round(E / t, 2),
+            "achieved_tflops": round(f * E / t / 1e12, 3),
+            "frac_fp64_peak": round(f * E / t / 1e12 / FP64_PEAK_TFLOPS, 4),
+            "ms_per_block_step": round(t * 1e3 / nb, 3)}
+
+
 def serving_leg(args, dev) -> dict:
     """Continuous batching over 6 resident geometries, Poisson arrivals, adaptive precision."""
     from quatrex_amd import serving as SV
@@ -565,7 +614,7 @@ def serving_leg(args, dev) -> dict:
 
 
 LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_sample", "dsdb_energies",
-            "obc_energies",
+            "obc_energies", "sections", "steps",
             "uneven_energies", "uneven_capacity", "uneven_budget_gib", "serving_requests",
             "serving_rate", "serving_budget_gib")
 
@@ -622,6 +671,8 @@ def leg_main(args) -> None:
         res = dsdb_leg(args, dev)
     elif args.only_leg == "obc":
         res = obc_leg(args, dev)
+    elif args.only_leg == "section":
+        res = section_leg(args, dev)
     else:
         res = serving_leg(args, dev)
     print(json.dumps(res), flush=True)
@@ -858,6 +909,8 @@ def main():
         legs.append(("block_sparse_io", "dsdb"))
     if args.obc_energies > 0:
         legs.append(("open_boundary", "obc"))
+    if args.sections > 1 and args.n_blocks // args.sections >= 2:
+        legs.append(("partition_section", "section"))
     if args.uneven_energies > 0:
         legs.append(("uneven_batching", "uneven"))
     if args.serving_requests > 0:
