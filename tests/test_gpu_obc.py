@@ -278,3 +278,55 @@ def test_compute_obc_with_overlap_matches_oracle(device):
     for k, lst in enumerate((obc.retarded, obc.lesser, obc.greater)):
         assert _rel(lst[0].cpu().numpy(), ref_l[k]) <= TOL
         assert _rel(lst[-1].cpu().numpy(), ref_r[k]) <= TOL
+
+
+def test_obc_entry_points_reject_bad_arguments(device):
+    """Invalid sizes and arguments come back as status codes (RgfError) or the
+    reference's exception types; nothing is launched for them."""
+    import ctypes
+
+    import torch
+
+    from quatrex_amd import _lib
+
+    lib = _lib.load()
+    ctx = S.default_context(device)
+    x = torch.zeros((2, 8, 8), dtype=torch.complex128, device=device)
+    it, d = ctypes.c_int32(), ctypes.c_double()
+    p = ctypes.c_void_p(x.data_ptr())
+    # block size out of range, negative energies, negative iterations, NULL buffers
+    assert lib.rgf_sancho_rubio_z(ctx.handle, 0, 2, p, p, p, 10, 1e-6, p, ctypes.byref(it),
+                                  ctypes.byref(d)) == _lib.RGF_EUNSUPPORTED
+    assert lib.rgf_sancho_rubio_z(ctx.handle, 8, -1, p, p, p, 10, 1e-6, p, ctypes.byref(it),
+                                  ctypes.byref(d)) == _lib.RGF_EINVAL
+    assert lib.rgf_sancho_rubio_z(ctx.handle, 8, 2, p, p, p, -1, 1e-6, p, ctypes.byref(it),
+                                  ctypes.byref(d)) == _lib.RGF_EINVAL
+    assert lib.rgf_sancho_rubio_z(ctx.handle, 8, 2, None, p, p, 10, 1e-6, p, ctypes.byref(it),
+                                  ctypes.byref(d)) == _lib.RGF_EINVAL
+    r = torch.zeros(2, dtype=torch.float64, device=device)
+    # x_ref aliasing x; one residual array without the other
+    assert lib.rgf_obc_fixed_point_z(ctx.handle, 8, 2, p, p, p, p, p, None, None) == _lib.RGF_EINVAL
+    x2 = torch.zeros_like(x)
+    assert lib.rgf_obc_fixed_point_z(ctx.handle, 8, 2, p, p, p, p, ctypes.c_void_p(x2.data_ptr()),
+                                     ctypes.c_void_p(r.data_ptr()), None) == _lib.RGF_EINVAL
+    assert lib.rgf_contact_obc_z(ctx.handle, 8, 2, p, p, p, None, p, p, p) == _lib.RGF_EINVAL
+    assert lib.rgf_boundary_system_z(ctx.handle, 8, 2, 0.1, None, p, p, None, None, None, p, None,
+                                     None) == _lib.RGF_EINVAL
+    # zero energies: nothing to do
+    assert lib.rgf_sancho_rubio_z(ctx.handle, 8, 0, p, p, p, 10, 1e-6, p, ctypes.byref(it),
+                                  ctypes.byref(d)) == _lib.RGF_OK
+    assert it.value == 0
+    # Python layer: mismatched shapes, non-square blocks, wrong occupancy count
+    y = torch.zeros((2, 6, 6), dtype=torch.complex128, device=device)
+    with pytest.raises(ValueError):
+        SanchoRubio()(x, y, x, "c")
+    with pytest.raises(ValueError):
+        SanchoRubio()(torch.zeros((2, 8, 6), dtype=torch.complex128, device=device), x, x, "c")
+    with pytest.raises(ValueError):
+        compute_contact_obc(x, x, x, np.zeros(3), 0.0, OBCSystem(SanchoRubio(), memoization_mode="off"))
+    with pytest.raises(ValueError):
+        OBCSystem(SanchoRubio(), agreement_threshold=1.5)
+    with pytest.raises(NotImplementedError):
+        OBCSystem(SanchoRubio(), system_reducer=object())
+    with pytest.warns(RuntimeWarning, match="at least 2"):
+        OBCSystem(SanchoRubio(), num_ref_iterations=1)
