@@ -606,6 +606,7 @@ def serving_leg(args, dev) -> dict:
         "hbm_budget_gib": args.serving_budget_gib,
         "slots": len(srv.slots),
         "slot_capacity": sorted({s.capacity for s in srv.slots}),
+        "gpu_max_hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "4 (HIP default)"),
         "tiles": srv.tiles,
         **{k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
         "below_capacity": {"offered_rps": round(rate2, 2),
@@ -627,8 +628,12 @@ def run_leg(leg: str, args, eps: float) -> dict:
            "--main-eps", repr(eps)]
     for name in LEG_ARGS:
         cmd += ["--" + name.replace("_", "-"), str(getattr(args, name))]
+    env = dict(os.environ)
+    if leg == "serving":
+        # one hardware queue per slot stream (12 slots); HIP reads this at start-up
+        env["GPU_MAX_HW_QUEUES"] = "16"
     try:
-        res = subprocess.run(cmd, capture_output=True, text=True, timeout=args.leg_timeout)
+        res = subprocess.run(cmd, capture_output=True, text=True, timeout=args.leg_timeout, env=env)
     except subprocess.TimeoutExpired:
         return {"error": f"timed out after {args.leg_timeout} s"}
     lines = [ln for ln in res.stdout.splitlines() if ln.startswith("{")]
