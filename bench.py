@@ -629,8 +629,8 @@ def run_leg(leg: str, args, eps: float) -> dict:
     for name in LEG_ARGS:
         cmd += ["--" + name.replace("_", "-"), str(getattr(args, name))]
     env = dict(os.environ)
-    if leg == "serving":
-        # one hardware queue per slot stream (12 slots); HIP reads this at start-up
+    if leg in ("serving", "uneven"):
+        # a hardware queue per slot stream (12 and 4 slots); HIP reads this at start-up
         env["GPU_MAX_HW_QUEUES"] = "16"
     try:
         res = subprocess.run(cmd, capture_output=True, text=True, timeout=args.leg_timeout, env=env)
