@@ -135,6 +135,25 @@ __global__ void shift_kernel(const double2* __restrict__ x, const double2* __res
   }
 }
 
+// eps <- eps + x and s <- eps (the inversion copy), per element of a chunk
+__global__ void eps_merge_kernel(double2* __restrict__ eps, const double2* __restrict__ x,
+                                 double2* __restrict__ s, int64_t count) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = cadd(eps[i], x[i]);
+    eps[i] = v;
+    s[i] = v;
+  }
+}
+
+hipError_t launch_eps_merge(double2* eps, const double2* x, double2* s, int64_t count,
+                            hipStream_t st) {
+  if (count == 0) return hipSuccess;
+  const int gx = (int)std::min<int64_t>(4096, (count + 255) / 256);
+  hipLaunchKernelGGL(eps_merge_kernel, dim3(gx), dim3(256), 0, st, eps, x, s, count);
+  return hipGetLastError();
+}
+
 hipError_t launch_shift(const double2* x, const double2* s, double eta, int b, int nE,
                         double2* out, hipStream_t st) {
   if (nE == 0) return hipSuccess;
@@ -268,17 +287,21 @@ int rgf_sancho_rubio_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, const void* 
           PB(n, n).term(M(Be[cur], e0), kOpN, Mat{G, bb, n}, kOpN, n).out(Mat{P2, bb, n}).done()};
       rc = ctx->gemm(p1, E, n);
       if (rc) return rc;
-      // epsilon -= alpha g beta + beta g alpha (copy into S for the next inverse),
-      // epsilon_s -= alpha g beta, alpha <- alpha g alpha, beta <- beta g beta
+      // epsilon_s -= alpha g beta, with -alpha g beta also written to G (free
+      // now: the inverse was last read above); epsilon -= beta g alpha;
+      // alpha <- alpha g alpha, beta <- beta g beta
       std::vector<ZProblem> p2 = {
           PB(n, n).term(Mat{P1, bb, n}, kOpN, M(Be[cur], e0), kOpN, n, MONE)
-              .term(Mat{P2, bb, n}, kOpN, M(Al[cur], e0), kOpN, n, MONE)
-              .add(M(Eps, e0), ONE).out(M(Eps, e0)).out2(kD2Copy, M(S, e0)).done(),
-          PB(n, n).term(Mat{P1, bb, n}, kOpN, M(Be[cur], e0), kOpN, n, MONE)
-              .add(M(EpsS, e0), ONE).out(M(EpsS, e0)).done(),
+              .partial_here().add(M(EpsS, e0), ONE).out(M(EpsS, e0))
+              .out2(kD2Partial, Mat{G, bb, n}).done(),
+          PB(n, n).term(Mat{P2, bb, n}, kOpN, M(Al[cur], e0), kOpN, n, MONE)
+              .add(M(Eps, e0), ONE).out(M(Eps, e0)).done(),
           PB(n, n).term(Mat{P1, bb, n}, kOpN, M(Al[cur], e0), kOpN, n).out(M(Al[nxt], e0)).done(),
           PB(n, n).term(Mat{P2, bb, n}, kOpN, M(Be[cur], e0), kOpN, n).out(M(Be[nxt], e0)).done()};
       rc = ctx->gemm(p2, E, n);
+      if (rc) return rc;
+      // epsilon -= alpha g beta; S <- epsilon for the next inverse
+      rc = ctx->hip(launch_eps_merge(Eps + e0 * bb, G, S + e0 * bb, (int64_t)E * bb, st), "eps merge");
       if (rc) return rc;
     }
     cur = nxt;
