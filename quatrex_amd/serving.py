@@ -237,7 +237,8 @@ class Server:
     seeds : list of int, model seeds per geometry
     budget_bytes : HBM bytes the server may use for slots
     capacity : max energies per batch (per slot; reduced to fit the budget)
-    slots_per_geometry : concurrent batches per geometry
+    slots_per_geometry : concurrent batches per geometry (one count for all, or
+        one per geometry)
     precision : "fp64", "mixed" or "adaptive"
     autotune : time the GEMM tile configurations per block size at start-up
         (cached on disk) and use the fastest in every slot
@@ -246,7 +247,7 @@ class Server:
     """
 
     def __init__(self, geometries, seeds, budget_bytes: int, capacity: int = 8,
-                 slots_per_geometry: int = 2, precision: str = "adaptive",
+                 slots_per_geometry=2, precision: str = "adaptive",
                  fallback_tol: float = 1e-8, refine: int = 1, retry_every: int = 8,
                  keep_outputs: set | None = None, use_graphs: bool = True, autotune: bool = False,
                  overlap: bool = False, device=None):
@@ -271,17 +272,22 @@ class Server:
             self.used += self._resident_bytes(g)
         if self.used > self.budget:
             raise AdmissionError(f"resident geometries need {self.used} bytes > budget {self.budget}")
+        counts = ([int(slots_per_geometry)] * len(geometries) if np.isscalar(slots_per_geometry)
+                  else [int(c) for c in slots_per_geometry])
+        if len(counts) != len(geometries) or min(counts) < 1:
+            raise ValueError("slots_per_geometry: one positive count, or one per geometry")
         for gid, g in enumerate(geometries):
             cap = capacity
-            need = slot_bytes(g, cap) * slots_per_geometry
+            nslots = counts[gid]
+            need = slot_bytes(g, cap) * nslots
             while cap > 1 and self.used + need > self.budget:
                 cap //= 2
-                need = slot_bytes(g, cap) * slots_per_geometry
+                need = slot_bytes(g, cap) * nslots
             if self.used + need > self.budget:
                 raise AdmissionError(
                     f"geometry {gid} ({g.n_blocks} blocks, max block {g.max_block}) needs "
-                    f"{need} bytes for {slots_per_geometry} slot(s); {self.budget - self.used} left")
-            for _ in range(slots_per_geometry):
+                    f"{need} bytes for {nslots} slot(s); {self.budget - self.used} left")
+            for _ in range(nslots):
                 self.slots.append(Slot(self, gid, cap, self.device, overlap=overlap))
             self.used += need
         streams = len(self.slots) * (2 if overlap else 1)

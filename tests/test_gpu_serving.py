@@ -75,6 +75,22 @@ def test_admission_by_footprint(device):
                slots_per_geometry=1, device=device)
 
 
+def test_slots_per_geometry_list_and_queue_warning(device, monkeypatch):
+    """A slot count per geometry; more slot streams than hardware queues warns."""
+    monkeypatch.setenv("GPU_MAX_HW_QUEUES", "4")
+    with pytest.warns(RuntimeWarning, match="hardware queues"):
+        srv = Server(GEOS, SEEDS, budget_bytes=8 << 30, capacity=2, slots_per_geometry=[1, 3, 1],
+                     precision="fp64", use_graphs=False, device=device)
+    try:
+        assert [sum(s.gid == k for s in srv.slots) for k in range(3)] == [1, 3, 1]
+        res = srv.run([Request(0.0, k, k % 3, 0.05 * k, 1e-3) for k in range(9)], realtime=False)
+        assert sorted(r.rid for r in res) == list(range(9))
+    finally:
+        srv.close()
+    with pytest.raises(ValueError):
+        Server(GEOS, SEEDS, budget_bytes=8 << 30, capacity=2, slots_per_geometry=[1, 1], device=device)
+
+
 def test_realtime_poisson_stream(device):
     from quatrex_amd.serving import poisson_requests
 
