@@ -62,8 +62,9 @@ def parse():
                     help="requests of the serving leg (a prefix of the Poisson stream; 0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=2000.0, help="offered load, requests/s")
     ap.add_argument("--serving-budget-gib", type=float, default=96.0)
-    ap.add_argument("--eta-sample", type=int, default=32,
-                    help="energies per eta of the 48-block mixed-precision error leg (0 = skip)")
+    ap.add_argument("--eta-energies", "--eta-sample", dest="eta_energies", type=int, default=1024,
+                    help="energies of the 48-block mixed-precision leg, eta cycled (0 = skip)")
+    ap.add_argument("--eta-chunk", type=int, default=128, help="energies per solve in that leg")
     ap.add_argument("--dsdb-energies", type=int, default=64,
                     help="energies of the block-sparse densify/sparsify leg (0 = skip)")
     ap.add_argument("--uneven-energies", type=int, default=2048,
@@ -253,12 +254,14 @@ def torch_leg(args, model, energies, eps, dev) -> dict:
 
 
 def eta_leg(args, model_seed, dev) -> dict:
-    """Mixed-precision error per output at 48 x 256 for eta in {1e-2, 1e-3, 1e-4}.
+    """Mixed precision at 48 x 256 over 1024 energies, eta cycled over
+    {1e-2, 1e-3, 1e-4} (energy k takes the (k mod 3)-th value).
 
-    For a sample of energies: the complex64-product solve (refine=1, no check)
-    against the FP64 solve of the same inputs, max over energies and blocks
-    of the relative Frobenius error per block; and the fraction of energies
-    that the FP64 check sends back to FP64 at the 1e-8 tolerance.
+    In chunks that fit HBM: the FP64 solve (timed), the complex64-product
+    solve with one FP64 refinement step and no check (timed), its error
+    against the FP64 result per output (max over energies and blocks of the
+    relative Frobenius error per block, per eta), and the mixed solve with the
+    FP64 check, whose fallback fraction at 1e-8 is reported per eta.
     """
     import torch
 
@@ -268,44 +271,76 @@ def eta_leg(args, model_seed, dev) -> dict:
 
     g = Geometry.uniform(48, args.block_size)
     model = W.make_device(g, model_seed)
-    ns = args.eta_sample
-    energies = W.draw_energies(ns)
-    out = {"workload": f"48 blocks x {args.block_size}, complex128, {ns} energies per eta",
-           "sample_energies": ns}
-    for eta in (1e-2, 1e-3, 1e-4):
-        A = W.system_matrix_torch(model, energies, eta, dev)
-        sld = torch.as_tensor(model.sl_diag, device=dev).expand(ns, -1).contiguous()
-        sgd = torch.as_tensor(model.sg_diag, device=dev).expand(ns, -1).contiguous()
-        slu = torch.zeros((ns, g.offdiag_size), dtype=torch.complex128, device=dev)
-        sgu = torch.zeros((ns, g.offdiag_size), dtype=torch.complex128, device=dev)
-        outs = (S.packed_empty(g, ns, dev), S.packed_empty(g, ns, dev, parts=("d", "u")),
-                S.packed_empty(g, ns, dev, parts=("d", "u")))
+    ne = args.eta_energies
+    etas = (1e-2, 1e-3, 1e-4)
+    energies = W.draw_energies(ne)
+    eta_of = np.array([etas[k % 3] for k in range(ne)])
+    ctx = S.default_context(dev)
+    errs = {eta: {"G^R": 0.0, "G^<": 0.0, "G^>": 0.0} for eta in etas}
+    fb = {eta: [] for eta in etas}
+    res = {eta: [] for eta in etas}
+    t64 = t32 = 0.0
+    for c0 in range(0, ne, args.eta_chunk):
+        sl = slice(c0, min(ne, c0 + args.eta_chunk))
+        n = sl.stop - sl.start
+        A = W.system_matrix_torch(model, energies[sl], eta_of[sl], dev)
+        sld = torch.as_tensor(model.sl_diag, device=dev).expand(n, -1).contiguous()
+        sgd = torch.as_tensor(model.sg_diag, device=dev).expand(n, -1).contiguous()
+        slu = torch.zeros((n, g.offdiag_size), dtype=torch.complex128, device=dev)
+        sgu = torch.zeros_like(slu)
+        outs = (S.packed_empty(g, n, dev), S.packed_empty(g, n, dev, parts=("d", "u")),
+                S.packed_empty(g, n, dev, parts=("d", "u")))
 
         def solve(**kw):
             S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_retarded=outs[0], g_lesser=outs[1],
-                             g_greater=outs[2], **kw)
-            S.default_context(dev).status()
+                             g_greater=outs[2], ctx=ctx, **kw)
 
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         solve()
+        torch.cuda.synchronize()
+        t64 += time.perf_counter() - t0
+        ctx.status()
         ref = [tuple(p.clone() for p in X if p is not None) for X in outs]
+        t0 = time.perf_counter()
         solve(precision="mixed", mixed=S.MixedOptions(refine=1, fallback_tol=0.0))
-        errs = {}
+        torch.cuda.synchronize()
+        t32 += time.perf_counter() - t0
+        ctx.status()
         for name, X, R in zip(("G^R", "G^<", "G^>"), outs, ref):
-            worst = 0.0
+            worst = torch.zeros(n, dtype=torch.float64, device=dev)
             for part, rp in zip([p for p in X if p is not None], R):
-                d = (part - rp).reshape(ns, -1, args.block_size ** 2)
-                r = rp.reshape(ns, -1, args.block_size ** 2)
-                worst = max(worst, float((torch.linalg.norm(d, dim=-1) / torch.linalg.norm(r, dim=-1)).max()))
-            errs[name] = float(f"{worst:.3e}")
-        flags = torch.zeros(ns, dtype=torch.int32, device=dev)
-        resid = torch.zeros(ns, dtype=torch.float64, device=dev)
+                d = (part - rp).reshape(n, -1, args.block_size ** 2)
+                r = rp.reshape(n, -1, args.block_size ** 2)
+                worst = torch.maximum(worst, (torch.linalg.norm(d, dim=-1) / torch.linalg.norm(r, dim=-1)).amax(-1))
+            w = worst.cpu().numpy()
+            for eta in etas:
+                m = eta_of[sl] == eta
+                if m.any():
+                    errs[eta][name] = max(errs[eta][name], float(w[m].max()))
+        flags = torch.zeros(n, dtype=torch.int32, device=dev)
+        resid = torch.zeros(n, dtype=torch.float64, device=dev)
         solve(precision="mixed", mixed=S.MixedOptions(refine=1, fallback_tol=1e-8, fallback_flags=flags,
                                                       residuals=resid))
-        out[f"eta={eta:g}"] = {"max_rel_err_per_block": errs,
-                               "fallback_fraction_at_1e-8": round(float(flags.float().mean()), 4),
-                               "median_residual": float(f"{float(resid.median()):.3e}")}
+        ctx.status()
+        f, rr = flags.cpu().numpy(), resid.cpu().numpy()
+        for eta in etas:
+            m = eta_of[sl] == eta
+            fb[eta].extend(f[m].tolist())
+            res[eta].extend(rr[m].tolist())
         del A, sld, sgd, slu, sgu, outs, ref
         torch.cuda.empty_cache()
+    f_e = flops_per_energy(48, args.block_size)
+    out = {"workload": f"48 blocks x {args.block_size}, complex128, {ne} energies, eta cycled over "
+                       f"{list(etas)}, chunks of {args.eta_chunk}",
+           "fp64": {"energies_per_s": round(ne / t64, 2),
+                    "frac_fp64_peak": round(f_e * ne / t64 / 1e12 / FP64_PEAK_TFLOPS, 4)},
+           "mixed_unchecked": {"energies_per_s": round(ne / t32, 2), "speedup_vs_fp64": round(t64 / t32, 3)}}
+    for eta in etas:
+        out[f"eta={eta:g}"] = {"energies": len(fb[eta]),
+                               "max_rel_err_per_block": {k: float(f"{v:.3e}") for k, v in errs[eta].items()},
+                               "fallback_fraction_at_1e-8": round(float(np.mean(fb[eta])), 4),
+                               "median_residual": float(f"{float(np.median(res[eta])):.3e}")}
     return out
 
 
@@ -614,7 +649,8 @@ def serving_leg(args, dev) -> dict:
     }
 
 
-LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_sample", "dsdb_energies",
+LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_energies", "eta_chunk",
+            "dsdb_energies",
             "obc_energies", "sections", "steps",
             "uneven_energies", "uneven_capacity", "uneven_budget_gib", "serving_requests",
             "serving_rate", "serving_budget_gib")
@@ -908,7 +944,7 @@ def main():
     legs = []
     if args.torch_sample > 0:
         legs.append(("torch_fp64", "torch"))
-    if args.eta_sample > 0 and not args.no_mixed:
+    if args.eta_energies > 0 and not args.no_mixed:
         legs.append(("mixed_error_vs_eta", "eta"))
     if args.dsdb_energies > 0:
         legs.append(("block_sparse_io", "dsdb"))
