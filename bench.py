@@ -73,9 +73,12 @@ def parse():
     ap.add_argument("--uneven-budget-gib", type=float, default=200.0)
     ap.add_argument("--obc-energies", type=int, default=512,
                     help="energies of the open-boundary (Sancho-Rubio) leg (0 = skip)")
+    ap.add_argument("--host-energies", type=int, default=64,
+                    help="energies of the host-resident (PCIe-inclusive) drop-in leg (0 = skip)")
     ap.add_argument("--sections", type=int, default=4,
                     help="block sections of the per-section (partitioned solve) leg (0 = skip)")
-    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving", "dsdb", "obc", "section"),
+    ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving", "dsdb", "obc", "section",
+                                           "host"),
                     default=None,
                     help=argparse.SUPPRESS)
     ap.add_argument("--main-eps", type=float, default=0.0, help=argparse.SUPPRESS)
@@ -605,6 +608,45 @@ def section_leg(args, dev) -> dict:
             "ms_per_block_step": round(t * 1e3 / nb, 3)}
 
 
+def host_io_leg(args, dev) -> dict:
+    """The drop-in RGF on host-resident (CPU torch) block-tridiagonal stacks:
+    blocks gathered on the host, staged through pinned memory to the device,
+    solved, and written back block by block - the PCIe-inclusive rate of a
+    caller that hands over host buffers (the device-resident rate is `value`)."""
+    import torch
+
+    from quatrex_amd import workloads as W
+    from quatrex_amd.datastructures import BlockTridiagonal
+    from quatrex_amd.greens_function_solver import RGF
+    from quatrex_amd.layout import Geometry
+
+    g = Geometry.uniform(args.n_blocks, args.block_size)
+    E = args.host_energies
+    model = W.make_device(g, W.DEVICE_SEED_BASE + 2)
+    ad, au, al = W.system_matrix(model, W.draw_energies(E), args.eta)
+    sld, slu, sgd, sgu = W.sigmas(model, E)
+    t = torch.from_numpy
+    bs = list(g.block_sizes)
+    A = BlockTridiagonal.from_packed(bs, t(ad), t(au), t(al))
+    # the workload's Sigma^<> have no off-diagonal blocks
+    SL = BlockTridiagonal.from_packed(bs, t(sld), t(slu), torch.zeros_like(t(slu)))
+    SG = BlockTridiagonal.from_packed(bs, t(sgd), t(sgu), torch.zeros_like(t(sgu)))
+    outs = tuple(BlockTridiagonal(bs, (E,)) for _ in range(3))
+    solver = RGF(max_batch_size=E)
+    solver.selected_solve(A, SL, SG, out=outs, return_retarded=True)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.selected_solve(A, SL, SG, out=outs, return_retarded=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    moved = 16 * E * (3 * (g.diag_size + 2 * g.offdiag_size) + 3 * (g.diag_size + 2 * g.offdiag_size))
+    return {"workload": f"{g.n_blocks} blocks x {args.block_size}, {E} energies, complex128, CPU-resident "
+                        "A, Sigma^<, Sigma^> and outputs (G^R, G^<, G^> all tridiagonal blocks)",
+            "energies_per_s": round(E / el, 2), "ms": round(el * 1e3, 1),
+            "host_bytes_in_and_out": moved,
+            "effective_gbs": round(moved / el / 1e9, 2)}
+
+
 def serving_leg(args, dev) -> dict:
     """Continuous batching over 6 resident geometries, Poisson arrivals, adaptive precision."""
     from quatrex_amd import serving as SV
@@ -651,7 +693,7 @@ def serving_leg(args, dev) -> dict:
 
 LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_energies", "eta_chunk",
             "dsdb_energies",
-            "obc_energies", "sections", "steps",
+            "obc_energies", "sections", "steps", "host_energies",
             "uneven_energies", "uneven_capacity", "uneven_budget_gib", "serving_requests",
             "serving_rate", "serving_budget_gib")
 
@@ -714,6 +756,8 @@ def leg_main(args) -> None:
         res = obc_leg(args, dev)
     elif args.only_leg == "section":
         res = section_leg(args, dev)
+    elif args.only_leg == "host":
+        res = host_io_leg(args, dev)
     else:
         res = serving_leg(args, dev)
     print(json.dumps(res), flush=True)
@@ -952,6 +996,8 @@ def main():
         legs.append(("open_boundary", "obc"))
     if args.sections > 1 and args.n_blocks // args.sections >= 2:
         legs.append(("partition_section", "section"))
+    if args.host_energies > 0:
+        legs.append(("host_resident_dropin", "host"))
     if args.uneven_energies > 0:
         legs.append(("uneven_batching", "uneven"))
     if args.serving_requests > 0:

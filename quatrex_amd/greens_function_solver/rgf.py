@@ -61,11 +61,13 @@ def _geometry(m) -> Geometry:
 def _gather(m, geo: Geometry, parts: str, device):
     """Packed ``(diag, upper, lower)`` device tensors of a stack view ``m``."""
     torch = _torch()
-    if _is_device_container(m) and not m.symmetry:
+    if isinstance(m, BlockTridiagonal) and not m.symmetry:
+        # the packed storage is already the device layout: used in place on
+        # the device, one transfer per part from host memory
         d, u, l = m.packed()
         out = []
         for key, t in zip("dul", (d, u, l)):
-            out.append(t.to(torch.complex128).contiguous() if key in parts else None)
+            out.append(t.to(device=device, dtype=torch.complex128).contiguous() if key in parts else None)
         return tuple(out)
     blk0 = m.blocks[0, 0]
     lead = tuple(blk0.shape[:-2])
@@ -119,6 +121,18 @@ def _outputs(m, geo: Geometry, E: int, device, parts: str):
 
 def _scatter(m, geo: Geometry, bufs, lead, blocks=("d", "u", "l")):
     """Write packed device results into a generic container view ``m``."""
+    if isinstance(m, BlockTridiagonal) and not m.symmetry:
+        # packed storage: one copy per part straight into it (a view is
+        # required: a reshaped copy would not reach the container)
+        try:
+            dsts = [t.view(-1, t.shape[-1]) for t in m.packed(flatten=False)]
+        except RuntimeError:
+            dsts = None
+        if dsts is not None:
+            for key, dst, src in zip("dul", dsts, bufs):
+                if key in blocks and src is not None:
+                    dst.copy_(src.reshape(dst.shape))
+            return
     d, u, l = (t.cpu() if t is not None else None for t in bufs)
     is_torch = isinstance(m, BlockTridiagonal)
 
