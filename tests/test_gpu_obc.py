@@ -330,3 +330,43 @@ def test_obc_entry_points_reject_bad_arguments(device):
         OBCSystem(SanchoRubio(), system_reducer=object())
     with pytest.warns(RuntimeWarning, match="at least 2"):
         OBCSystem(SanchoRubio(), num_ref_iterations=1)
+
+
+@pytest.mark.parametrize("layout_name", ["coo", "csr"])
+def test_boundary_conditions_and_solve_on_block_sparse_stacks(device, layout_name):
+    """The electron-solver step on the reference's storage: boundary
+    self-energies of both contacts read from a block-sparse system matrix
+    (compute_obc through .blocks), then the drop-in RGF on block-sparse
+    stacks with those OBC blocks and the current, against the dense oracle
+    with the same boundary blocks."""
+    from oracle.dense import dense_solve
+    from quatrex_amd.dsdb import DSDBCOO, DSDBCSR
+    from tests.helpers import block_rel_errors, tridiagonal_mask
+    from tests.test_gpu_dsdb import _container
+
+    layout = DSDBCOO if layout_name == "coo" else DSDBCSR
+    b, N = 10, 5
+    energies = np.array([0.05, -0.25, 0.4])
+    E = len(energies)
+    A, off = periodic_device(b, N, energies, seed=41)
+    A = A + 1j * 1e-3 * np.eye(b * N)
+    bs = [b] * N
+    zero = np.zeros_like(A)
+    skew = lambda x: -np.conj(x)  # noqa: E731
+    a = _container(layout, A, bs, False, np.conj, device)
+    sl = _container(layout, zero, bs, True, skew, device)
+    sg = _container(layout, zero, bs, True, skew, device)
+    fL, fR = np.array([0.9, 0.6, 0.3]), np.array([0.1, 0.4, 0.2])
+    obc = compute_obc(a, fL, fR, 1e-3, OBCSystem(SanchoRubio(100, 1e-10), memoization_mode="off"))
+    xl, xg, xr, cur = RGF(max_batch_size=2).selected_solve(a, sl, sg, obc_blocks=obc,
+                                                          return_retarded=True, return_current=True)
+    obc_np = tuple([None if t is None else t.cpu().numpy() for t in lst]
+                   for lst in (obc.retarded, obc.lesser, obc.greater))
+    ref = dense_solve(A, zero, zero, off, obc_np)
+    mask = tridiagonal_mask(bs)
+    for X, R, name in zip((xr, xl, xg), ref, ("G^R", "G^<", "G^>")):
+        err = block_rel_errors(X.to_dense().cpu().numpy(), R * mask, bs)
+        assert err <= 1e-10, (name, err)
+    cur = cur.cpu().numpy() if hasattr(cur, "cpu") else np.asarray(cur)
+    ref_cur = dense_current(A, zero, zero, off, obc_np)
+    np.testing.assert_allclose(cur, ref_cur, rtol=0, atol=1e-8 * np.abs(ref_cur).max())
