@@ -77,40 +77,6 @@ struct ZGroup {
   ZProblem p[kMaxProblems];
 };
 
-// Single-precision twin of ZProblem for the mixed-precision sweeps: operands
-// are complex64 (float2), accumulation is FP32 on MFMA.
-struct CTerm {
-  const float2* A;
-  const float2* B;
-  int64_t sA, sB;
-  int32_t lda, ldb, opA, opB, k, pad_;
-  float2 alpha;
-};
-struct CAddend {
-  const float2* F;
-  int64_t sF;
-  int32_t ldf, pad_;
-  float2 c1, c2;
-};
-struct CProblem {
-  int32_t m, n, nterms, naddends, npre, d2mode;
-  CTerm t[2];
-  CAddend ad[kMaxAddends];
-  float2* D;
-  int64_t sD;
-  int32_t ldd, ldd2;
-  float2* D2;
-  int64_t sD2;
-  float2 s2;
-};
-struct CGroup {
-  int32_t nprob, nE;
-  int32_t wg_base[kMaxProblems + 1];
-  int32_t tiles_n[kMaxProblems];
-  int32_t tiles_mn[kMaxProblems];
-  CProblem p[kMaxProblems];
-};
-
 // Tile configurations selectable per launch (keyed by block size by the autotuner).
 enum TileCfg : int32_t {
   kTile64x64 = 0,  // 256 threads, 4 waves of 32x32
@@ -121,6 +87,9 @@ enum TileCfg : int32_t {
 };
 int tile_m(int cfg);
 int tile_n(int cfg);
+// Configurations the FP32-compute products are built for (the others fall
+// back to 64x64 there).
+inline bool f32_tile_ok(int cfg) { return cfg == kTile64x64 || cfg == kTile32x32; }
 
 // XCD-aware remap: hardware dispatches consecutive workgroups round-robin over
 // the 8 XCDs; give each XCD a contiguous range of logical ids so tiles of one
@@ -136,7 +105,6 @@ hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream);
 hipError_t launch_zgemm_update(const ZGroup& g, int cfg, hipStream_t stream);
 // Same problems, products computed in complex64 on FP32 matrix cores.
 hipError_t launch_zgemm_f32(const ZGroup& g, int cfg, hipStream_t stream);
-hipError_t launch_cgemm(const CGroup& g, int cfg, hipStream_t stream);
 
 // Batched Gauss-Jordan inversion with partial pivoting, in place in S
 // (n x n, ld n, energy stride sS); result (column-permuted back) written to

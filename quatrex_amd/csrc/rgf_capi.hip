@@ -999,7 +999,7 @@ int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value) {
         const bool f32 = option >= 3000;
         const int b = option - (f32 ? 3000 : 1000);
         if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
-        if (f32 && value != qx::kTile32x32 && value != qx::kTile64x64 && value >= 0)
+        if (f32 && value >= 0 && !qx::f32_tile_ok((int)value))
           return ctx->fail(RGF_EINVAL, "FP32-compute products support tile configs 0 and 1");
         std::vector<int>& t = f32 ? ctx->tile_table32 : ctx->tile_table;
         if ((int)t.size() <= b) t.resize(b + 1, -1);
@@ -1261,7 +1261,7 @@ int rgf_autotune(rgf_ctx* ctx, int64_t b, int64_t n_energies, int32_t f32, int32
   for (int cfg = 0; cfg < qx::kNumTileCfgs; ++cfg) tmin[cfg] = -1.f;
   for (int round = 0; round < 5 && !rc; ++round) {
     for (int cfg = 0; cfg < qx::kNumTileCfgs && !rc; ++cfg) {
-      const bool ok = f32 ? (cfg == qx::kTile32x32 || cfg == qx::kTile64x64) : true;
+      const bool ok = f32 ? qx::f32_tile_ok(cfg) : true;
       if (!ok) continue;
       ctx->tile_override = cfg;
       if (round == 0) rc = ctx->gemm(probs, (int)n_energies, (int)b);  // warm-up

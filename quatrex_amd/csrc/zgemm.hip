@@ -344,7 +344,10 @@ zgemm_grouped_kernel(const ZGroup g) {
     }
     if (P.d2mode == kD2ConjT) {
       // D2[c][r] = s2 conj(D[r][c]): read the slab transposed from LDS, store
-      // runs of kSlab consecutive elements of rows of D2
+      // runs of kSlab consecutive elements of rows of D2.  The transposed read
+      // takes elements that other waves finished in the addend pass: wait for
+      // them (the condition is uniform over the workgroup).
+      __syncthreads();
 #pragma unroll 4
       for (int q = 0; q < kPer; ++q) {
         const int idx = tid + q * S::kThreads;

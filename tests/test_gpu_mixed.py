@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from oracle.dense import dense_solve
+from quatrex_amd import _lib
 from quatrex_amd import solve as S
 from quatrex_amd import workloads as W
 from quatrex_amd.layout import Geometry
@@ -81,6 +82,22 @@ def test_mixed_error_is_fp32_level(device, eta):
     errs = _errors(g, outs, A, SL, SG)
     assert max(errs) < 1e-2, errs
     assert max(errs) > 1e-12, errs  # it really ran in reduced precision
+
+
+@pytest.mark.parametrize("cfg", [0, 1])
+def test_mixed_every_f32_tile_config(device, cfg):
+    """Each tile configuration built for the FP32-compute products (64x64,
+    32x32) gives FP32-level results at b = 200 (partial tiles)."""
+    g, A, SL, SG, t = _case(device, 4, 200, 2, 1e-2, 3004)
+    ctx = S.default_context(device)
+    ctx.set_option(_lib.RGF_OPT_TILE_CFG, cfg)
+    try:
+        outs = _run(device, g, A, SL, SG, t, "mixed", S.MixedOptions(refine=1, fallback_tol=0.0))
+    finally:
+        ctx.set_option(_lib.RGF_OPT_TILE_CFG, -1)
+    errs = _errors(g, outs, A, SL, SG)
+    assert max(errs) < 1e-3, errs
+    assert max(errs) > 1e-12, errs
 
 
 def test_mixed_residual_matches_cpu_and_flags_fallback(device):

@@ -12,6 +12,7 @@ import pytest
 from oracle.dense import dense_current, dense_solve
 from oracle.rgf_ref import BlockTri
 from oracle.rgf_ref import selected_solve as oracle_solve
+from quatrex_amd import _lib
 from quatrex_amd import solve as S
 from quatrex_amd import workloads as W
 from quatrex_amd.datastructures import BlockTridiagonal
@@ -237,6 +238,40 @@ def test_mixed_blocks_offdiag_sigma_obc_current(device):
     L, R = one(bs[0], 0.8), one(bs[-1], 0.3)
     obc = [[L[k]] + [None] * (len(bs) - 2) + [R[k]] for k in range(3)]
     errs = _check_solve(device, g, A, SL, SG, obc_np=obc, with_current=True)
+    assert max(errs) <= TOL, errs
+
+
+@pytest.mark.parametrize("cfg", range(_lib.RGF_NUM_TILE_CFGS))
+def test_every_tile_config_vs_dense(device, cfg):
+    """Every GEMM tile configuration the autotuner can install gives the dense
+    result: block sizes that leave partial tiles of each shape (24 ... 200),
+    off-diagonal Sigma, both contacts and the current."""
+    rng = np.random.default_rng(400 + cfg)
+    bs = [24, 40, 130, 33, 200]
+    g = Geometry(tuple(bs))
+    model = W.make_device(g, 78)
+    en = np.array([0.2, -0.45])
+    A = W.system_matrix(model, en, 1e-2, fold_boundary=False)
+    E = len(en)
+    sl = np.broadcast_to(skew_tridiagonal(bs, rng), (E, g.n, g.n))
+    sg = np.broadcast_to(skew_tridiagonal(bs, rng), (E, g.n, g.n))
+    SL = g.from_dense(sl)[:2]
+    SG = g.from_dense(sg)[:2]
+
+    def one(b, f):
+        br = -0.5j * np.eye(b) + 0.05 * (rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b)))
+        gam = 1j * (br - br.conj().T)
+        rep = lambda x: np.broadcast_to(x, (E, b, b)).copy()  # noqa
+        return rep(br), rep(1j * gam * f), rep(1j * gam * (f - 1))
+
+    L, R = one(bs[0], 0.7), one(bs[-1], 0.2)
+    obc = [[L[k]] + [None] * (len(bs) - 2) + [R[k]] for k in range(3)]
+    ctx = S.default_context(device)
+    ctx.set_option(_lib.RGF_OPT_TILE_CFG, cfg)
+    try:
+        errs = _check_solve(device, g, A, SL, SG, obc_np=obc, with_current=True)
+    finally:
+        ctx.set_option(_lib.RGF_OPT_TILE_CFG, -1)
     assert max(errs) <= TOL, errs
 
 
