@@ -4,17 +4,30 @@
 // Step c eliminates column c with pivot row pi(c), the row of largest
 // |re|+|im| among the rows not yet used as pivots (LAPACK izamax's measure,
 // first physical row on ties).  Nothing is swapped: after all steps the matrix
-// holds M with A^{-1}[i][j] = M[pi(i)][pi^{-1}(j)].  Per panel of NB columns
-// [k, k+nb):
-//   1. zinv_panel_kernel (one workgroup per energy, one physical row per
-//      thread, the row's panel part in registers): unblocked GJE on the panel
-//      with the pivot search by wavefront shuffle max-reduction.  Writes the
-//      transformed panel T in place and the pivot rows, moves the pivot rows'
-//      non-panel columns to Wp (nb x n) and zeroes them in S.
-//   2. Trailing update on MFMA (grouped GEMM, two column ranges):
-//      S[:, nonpanel] += T * Wp[:, nonpanel].
-//      This is the blocked GJE update S += (T - J) Wp with J[pi(k+q)][q] = 1:
-//      J Wp restores the pivot rows, which the move to Wp left at zero.
+// holds M with A^{-1}[i][j] = M[pi(i)][pi^{-1}(j)].  A panel of NB columns
+// [k, k+nb) is eliminated by zinv_panel_kernel (one workgroup per energy, one
+// physical row per thread, the row's panel part in registers): unblocked GJE
+// on the panel with the pivot search by wavefront shuffle max-reduction.  It
+// writes the transformed panel T in place, moves the pivot rows' non-panel
+// columns to Wp (nb x n) and zeroes them in S.  The blocked GJE update is
+//   S[:, nonpanel] += T * Wp[:, nonpanel],
+// i.e. S += (T - J) Wp with J[pi(k+q)][q] = 1: J Wp restores the pivot rows,
+// which the move to Wp left at zero.
+//
+// The update reads and writes all of S and bounds the inversion by HBM, so
+// panels are taken in pairs (P1, P2) and the columns R outside both are
+// updated once, with K = nb1 + nb2:
+//   1. panel P1 -> T1, WA
+//   2. S[:, P2] += T1 WA[:, P2]                   (P2 columns only)
+//   3. panel P2 -> T2, WB; the move zeroes rows pi(P2) of T1 as well, and
+//      WB[:, P1] holds those rows of T1
+//   4. WB[:, R] += WB[:, P1] WA[:, R]             (WB's R columns were read
+//                                                   before P1's update)
+//   5. S[:, R] += T1 WA[:, R] + T2 WB[:, R]       (one pass over R)
+//   6. S[:, P1] += T2 WB[:, P1]
+// Row by row this is the sequence of the two single-panel updates: rows
+// pi(P1) and pi(P2) are zero in S[:, R] before step 5, and rows pi(P2) of T1
+// are zero there, so their P1 contribution enters only through WB.
 // zinv_finish_kernel then writes Out[i][j] = S[pi(i)][pi^{-1}(j)].
 #include "qx_common.h"
 
@@ -24,10 +37,13 @@ namespace {
 
 template <int NB>
 struct PanelAux {
-  // byte offsets inside one energy's aux area
-  __host__ __device__ static size_t wp(int) { return 0; }
+  // byte offsets inside one energy's aux area: two pivot-row buffers (the
+  // panels of a pair), then the pivot bookkeeping
+  __host__ __device__ static size_t wp(int n, int slot) {
+    return (size_t)slot * n * NB * sizeof(double2);
+  }
   // pivot row of each step (pi)
-  __host__ __device__ static size_t piv(int n) { return wp(n) + (size_t)n * NB * sizeof(double2); }
+  __host__ __device__ static size_t piv(int n) { return wp(n, 2); }
   // step at which each row was the pivot (pi^{-1}; -1 while unused)
   __host__ __device__ static size_t step(int n) { return piv(n) + (size_t)n * sizeof(int); }
   __host__ __device__ static size_t bytes(int n) {
@@ -55,11 +71,11 @@ struct PanelShared {
 template <int NB, int MAXT>
 __global__ void __launch_bounds__(MAXT)
 zinv_panel_kernel(double2* S, int64_t sS, int n, int k, int nbe, char* aux, int64_t sAux,
-                  int* info, int info_tag, int64_t e0) {
+                  int wp_slot, int* info, int info_tag, int64_t e0) {
   const int e = blockIdx.x;
   double2* A = S + (int64_t)e * sS;
   char* ax = aux + (int64_t)e * sAux;
-  double2* Wp = reinterpret_cast<double2*>(ax + PanelAux<NB>::wp(n));
+  double2* Wp = reinterpret_cast<double2*>(ax + PanelAux<NB>::wp(n, wp_slot));
   int* piv = reinterpret_cast<int*>(ax + PanelAux<NB>::piv(n));
   int* step_of = reinterpret_cast<int*>(ax + PanelAux<NB>::step(n));
 
@@ -253,31 +269,78 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
                     hipStream_t stream) {
   char* ax = static_cast<char*>(aux);
   const int64_t sAux = (int64_t)PanelAux<NB>::bytes(n);
+  const int64_t sAuxC = sAux / (int64_t)sizeof(double2);
+  double2* wa = reinterpret_cast<double2*>(ax + PanelAux<NB>::wp(n, 0));
+  double2* wb = reinterpret_cast<double2*>(ax + PanelAux<NB>::wp(n, 1));
   const int threads = ((n + 63) / 64) * 64;
-  for (int k = 0; k < n; k += NB) {
-    const int nbe = (n - k < NB) ? (n - k) : NB;
+  auto panel = [&](int k, int nbe, int slot) {
     hipLaunchKernelGGL((zinv_panel_kernel<NB, MAXT>), dim3(nE), dim3(threads), 0, stream, S, sS, n,
-                       k, nbe, ax, sAux, info, info_tag, e0);
-    if (nbe == n) break;  // single panel: nothing else to update
-    ZGroup g{};
-    g.nE = nE;
-    int np = 0;
-    const int64_t sAuxC = sAux / (int64_t)sizeof(double2);
-    double2* wp = reinterpret_cast<double2*>(ax + PanelAux<NB>::wp(n));
-    auto add = [&](int c0, int nc) {
-      if (nc <= 0) return;
-      ZProblem& P = g.p[np++];
-      P.m = n; P.n = nc; P.nterms = 1; P.naddends = 1; P.npre = 1; P.d2mode = kD2None;
-      P.t[0] = ZTerm{S + k, wp + c0, sS, sAuxC, n, n, kOpN, kOpN, nbe, 0, cmk(1.0, 0.0)};
-      P.ad[0] = ZAddend{S + c0, sS, n, 0, cmk(1.0, 0.0), cmk(0.0, 0.0)};
-      P.D = S + c0; P.sD = sS; P.ldd = n;
-    };
-    add(0, k);
-    add(k + nbe, n - k - nbe);
-    g.nprob = np;
-    hipError_t err = launch_zgemm_update(g, tile_cfg, stream);
-    if (err != hipSuccess) return err;
+                       k, nbe, ax, sAux, slot, info, info_tag, e0);
+  };
+  // operands: columns c0.. of S (stride sS) or of a pivot-row buffer (stride sAuxC), ld n
+  auto tS = [&](int c0, const double2* B, int k) {
+    return ZTerm{S + c0, B, sS, sAuxC, n, n, kOpN, kOpN, k, 0, cmk(1.0, 0.0)};
+  };
+  // D (m x nc at D, per-energy stride sD, ld n) += sum of the terms
+  auto add = [&](ZGroup& g, int m, int nc, double2* D, int64_t sD, ZTerm t0, const ZTerm* t1) {
+    if (m <= 0 || nc <= 0) return;
+    ZProblem& P = g.p[g.nprob++];
+    P.m = m; P.n = nc; P.nterms = t1 ? 2 : 1; P.naddends = 1; P.npre = 1; P.d2mode = kD2None;
+    P.t[0] = t0;
+    if (t1) P.t[1] = *t1;
+    P.ad[0] = ZAddend{D, sD, n, 0, cmk(1.0, 0.0), cmk(0.0, 0.0)};
+    P.D = D; P.sD = sD; P.ldd = n;
+  };
+  auto run = [&](ZGroup& g) { return g.nprob ? launch_zgemm_update(g, tile_cfg, stream) : hipSuccess; };
+  hipError_t err = hipSuccess;
+  for (int k = 0; k < n;) {
+    const int nb1 = (n - k < NB) ? (n - k) : NB;
+    const int k2 = k + nb1;
+    const int nb2 = (n - k2 < NB) ? (n - k2) : NB;
+    panel(k, nb1, 0);
+    if (nb1 == n) break;  // single panel: nothing else to update
+    if (nb2 == 0) {       // last panel, unpaired: columns before it
+      ZGroup g{};
+      g.nE = nE;
+      add(g, n, k, S, sS, tS(k, wa, nb1), nullptr);
+      err = run(g);
+      break;
+    }
+    const int k3 = k2 + nb2;  // R = [0, k) and [k3, n)
+    {
+      ZGroup g{};
+      g.nE = nE;
+      add(g, n, nb2, S + k2, sS, tS(k, wa + k2, nb1), nullptr);
+      if ((err = run(g)) != hipSuccess) return err;
+    }
+    panel(k2, nb2, 1);
+    {
+      ZGroup g{};
+      g.nE = nE;
+      const ZTerm a0{wb + k, wa, sAuxC, sAuxC, n, n, kOpN, kOpN, nb1, 0, cmk(1.0, 0.0)};
+      ZTerm a1 = a0;
+      a1.B = wa + k3;
+      add(g, nb2, k, wb, sAuxC, a0, nullptr);
+      add(g, nb2, n - k3, wb + k3, sAuxC, a1, nullptr);
+      if ((err = run(g)) != hipSuccess) return err;
+    }
+    {
+      ZGroup g{};
+      g.nE = nE;
+      const ZTerm l2 = tS(k2, wb, nb2), r2 = tS(k2, wb + k3, nb2);
+      add(g, n, k, S, sS, tS(k, wa, nb1), &l2);
+      add(g, n, n - k3, S + k3, sS, tS(k, wa + k3, nb1), &r2);
+      if ((err = run(g)) != hipSuccess) return err;
+    }
+    {
+      ZGroup g{};
+      g.nE = nE;
+      add(g, n, nb1, S + k, sS, tS(k2, wb + k, nb2), nullptr);
+      if ((err = run(g)) != hipSuccess) return err;
+    }
+    k = k3;
   }
+  if (err != hipSuccess) return err;
   const int rows = kFinishElems / n;  // n <= 1024: at least 2
   hipLaunchKernelGGL((zinv_finish_kernel<NB>), dim3((n + rows - 1) / rows, nE), dim3(256), 0,
                      stream, S, sS, n, ax, sAux, Out, sOut, ldo, rows);
