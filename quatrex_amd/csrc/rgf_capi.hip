@@ -1072,8 +1072,9 @@ int rgf_inv_z(rgf_ctx* ctx, int64_t n, int64_t n_mats, const void* a, void* out)
   int rc = hipSetDevice(ctx->device) == hipSuccess ? RGF_OK : RGF_EHIP;
   if (rc) return ctx->fail(rc, "hipSetDevice");
   const size_t per = (size_t)n * n * sizeof(double2) + qx::zinv_aux_bytes((int)n);
-  int64_t chunk = std::max<int64_t>(1, (int64_t)(ctx->ws_budget / per));
-  if (!ctx->ws_owned && ctx->ws) chunk = std::max<int64_t>(1, (int64_t)(ctx->ws_bytes / per));
+  // the workspace holds chunk matrices plus 256 bytes of alignment slack
+  const size_t budget = (!ctx->ws_owned && ctx->ws) ? ctx->ws_bytes : ctx->ws_budget;
+  int64_t chunk = std::max<int64_t>(1, (int64_t)((budget > 256 ? budget - 256 : 0) / per));
   chunk = std::min<int64_t>({chunk, n_mats, 65535});
   char* ws = nullptr;
   rc = ctx->ensure_workspace(per * chunk + 256, &ws);

@@ -308,6 +308,83 @@ def test_energy_alone_equals_in_batch(device):
             assert torch.equal(pb[2:3], pa)
 
 
+def test_caller_workspace_solve_exact_and_chunked(device):
+    """A caller-owned workspace of exactly rgf_workspace_size bytes for all
+    energies, or for 2 of the 5 (the sweep then runs in chunks), gives a
+    context-owned workspace's results bit for bit; one byte less than one
+    energy needs raises."""
+    import torch
+
+    g = Geometry.uniform(5, 48)
+    _, A, SL, SG = _generator_case(g, 5, 61)
+    t = lambda x: _t(x, device)  # noqa
+
+    def run(ctx):
+        out = [S.packed_empty(g, 5, device) for _ in range(3)]
+        S.selected_solve(g, tuple(map(t, A)), tuple(map(t, SL)), tuple(map(t, SG)),
+                         g_lesser=out[1], g_greater=out[2], g_retarded=out[0], ctx=ctx)
+        ctx.status()
+        torch.cuda.synchronize()
+        return out
+
+    ref_ctx = S.Context(device)
+    try:
+        ref = run(ref_ctx)
+    finally:
+        ref_ctx.close()
+    for ne in (5, 2):
+        ctx = S.Context(device)
+        try:
+            ctx.set_workspace(torch.empty(S.workspace_size(g, ne), dtype=torch.uint8, device=device))
+            out = run(ctx)
+        finally:
+            ctx.close()
+        for X, R in zip(out, ref):
+            for p, q in zip(X, R):
+                assert torch.equal(p, q), ne
+    ctx = S.Context(device)
+    try:
+        ctx.set_workspace(torch.empty(S.workspace_size(g, 1) - 1, dtype=torch.uint8, device=device))
+        with pytest.raises(_lib.RgfError):
+            run(ctx)
+    finally:
+        ctx.close()
+
+
+def test_caller_workspace_inverse_sizes(device):
+    """rgf_inv_z on caller-owned workspaces of every size from one matrix to
+    about three (256-byte steps): once a size is large enough for one matrix
+    and its auxiliaries, every larger size works (exact multiples of the
+    per-matrix need included) and the result, in chunks, matches numpy."""
+    import torch
+
+    n, nm = 64, 7
+    rng = np.random.default_rng(64)
+    a = rng.standard_normal((nm, n, n)) + 1j * rng.standard_normal((nm, n, n)) + 8 * np.eye(n)
+    ref = np.linalg.inv(a)
+    ad = _t(a, device)
+    big = torch.empty(3 * (2 * n * n * 16 + 4096), dtype=torch.uint8, device=device)
+    ctx = S.Context(device)
+    first_ok = None
+    try:
+        for nbytes in range(n * n * 16, big.numel() + 1, 256):
+            ctx.set_workspace(big[:nbytes])
+            try:
+                out = S.inv(ad, ctx=ctx)
+                ctx.status()
+            except _lib.RgfError:
+                assert first_ok is None, (nbytes, first_ok)
+                continue
+            if first_ok is None:
+                first_ok = nbytes
+            if (nbytes - first_ok) % 8192 == 0:
+                err = np.abs(out.cpu().numpy() - ref).max() / np.abs(ref).max()
+                assert err < 1e-13, (nbytes, err)
+    finally:
+        ctx.close()
+    assert first_ok is not None
+
+
 @pytest.mark.slow
 def test_full_size_32x256_identities(device):
     """32 x 256: block-row identity (A G^R)_ii = I, anti-Hermitian G^<, and a
