@@ -77,8 +77,11 @@ def parse():
                     help="energies of the host-resident (PCIe-inclusive) drop-in leg (0 = skip)")
     ap.add_argument("--sections", type=int, default=4,
                     help="block sections of the per-section (partitioned solve) leg (0 = skip)")
+    ap.add_argument("--small-energies", type=int, default=4096,
+                    help="energies of the small-block (b = 32) shapes of the HBM-bound leg; b = 64 "
+                         "runs a quarter of them (0 = skip)")
     ap.add_argument("--only-leg", choices=("torch", "eta", "uneven", "serving", "dsdb", "obc", "section",
-                                           "host"),
+                                           "host", "small"),
                     default=None,
                     help=argparse.SUPPRESS)
     ap.add_argument("--main-eps", type=float, default=0.0, help=argparse.SUPPRESS)
@@ -608,6 +611,78 @@ def section_leg(args, dev) -> dict:
             "ms_per_block_step": round(t * 1e3 / nb, 3)}
 
 
+def small_block_leg(args, dev) -> dict:
+    """Small blocks, where a complex product's intensity (about b/6 flop per
+    byte with its operands in HBM) is below the FP64 ridge (78.6 / 8 = 9.8):
+    the FP64 selected solve over 8 x 32 (the one-energy CPU configuration's
+    shape, here with many energies), 32 x 32 and 32 x 64.  Energies/s from
+    plain steps; GEMM time, flops and algorithmic bytes (each operand of each
+    problem read once, each output written once) from one step with the
+    library's HIP-event profiling on."""
+    import torch
+
+    from quatrex_amd import autotune
+    from quatrex_amd import solve as S
+    from quatrex_amd import workloads as W
+    from quatrex_amd.layout import Geometry
+
+    out = {"definition": "GEMM achieved_gbs = algorithmic operand and output bytes of the sweep "
+                         "products / their HIP-event time; intensity = their flops / those bytes"}
+    ctx = S.default_context(dev)
+    for nb, b in ((8, 32), (32, 32), (32, 64)):
+        E = args.small_energies if b <= 32 else max(1, args.small_energies // 4)
+        g = Geometry.uniform(nb, b)
+        model = W.make_device(g, W.DEVICE_SEED_BASE + 2)
+        A = W.system_matrix_torch(model, W.draw_energies(E), args.eta, dev)
+        sld = torch.as_tensor(model.sl_diag, device=dev).expand(E, -1).contiguous()
+        sgd = torch.as_tensor(model.sg_diag, device=dev).expand(E, -1).contiguous()
+        slu = torch.zeros((E, g.offdiag_size), dtype=torch.complex128, device=dev)
+        sgu = torch.zeros_like(slu)
+        GR = S.packed_empty(g, E, dev)
+        GL = S.packed_empty(g, E, dev, parts=("d", "u"))
+        GG = S.packed_empty(g, E, dev, parts=("d", "u"))
+        tile = autotune.tune(ctx, b, n_energies=min(E, 1024))["best_name"]
+
+        def step():
+            S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG,
+                             g_retarded=GR, ctx=ctx)
+
+        step()
+        ctx.status()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / args.steps
+        ctx.status()
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        step()
+        prof = ctx.profile_read(reset=True)
+        ctx.profile(False)
+        ctx.status()
+        f = flops_per_energy(nb, b)
+        gms = max(prof["gemm_ms"], 1e-9)
+        gbs = prof["gemm_bytes"] / gms / 1e6
+        out[f"{nb}x{b}"] = {
+            "energies": E, "tile": tile,
+            "energies_per_s": round(E / t, 1),
+            "sweep_tflops": round(f * E / t / 1e12, 3),
+            "sweep_frac_fp64_peak": round(f * E / t / 1e12 / FP64_PEAK_TFLOPS, 4),
+            "gemm": {"launches": prof["gemm_launches"],
+                     "achieved_tflops": round(prof["gemm_flops"] / gms / 1e9, 3),
+                     "achieved_gbs": round(gbs, 1),
+                     "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                     "intensity_flop_per_byte": round(prof["gemm_flops"] / max(prof["gemm_bytes"], 1), 2)},
+            "inverse_ms_share": round(prof["inv_ms"] / max(prof["inv_ms"] + prof["gemm_ms"], 1e-9), 4),
+        }
+        del A, sld, sgd, slu, sgu, GR, GL, GG
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    return out
+
+
 def host_io_leg(args, dev) -> dict:
     """The drop-in RGF on host-resident (CPU torch) block-tridiagonal stacks:
     blocks gathered on the host, staged through pinned memory to the device,
@@ -693,7 +768,7 @@ def serving_leg(args, dev) -> dict:
 
 LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_energies", "eta_chunk",
             "dsdb_energies",
-            "obc_energies", "sections", "steps", "host_energies",
+            "obc_energies", "sections", "steps", "host_energies", "small_energies",
             "uneven_energies", "uneven_capacity", "uneven_budget_gib", "serving_requests",
             "serving_rate", "serving_budget_gib")
 
@@ -758,6 +833,8 @@ def leg_main(args) -> None:
         res = section_leg(args, dev)
     elif args.only_leg == "host":
         res = host_io_leg(args, dev)
+    elif args.only_leg == "small":
+        res = small_block_leg(args, dev)
     else:
         res = serving_leg(args, dev)
     print(json.dumps(res), flush=True)
@@ -998,6 +1075,8 @@ def main():
         legs.append(("partition_section", "section"))
     if args.host_energies > 0:
         legs.append(("host_resident_dropin", "host"))
+    if args.small_energies > 0:
+        legs.append(("small_blocks", "small"))
     if args.uneven_energies > 0:
         legs.append(("uneven_batching", "uneven"))
     if args.serving_requests > 0:

@@ -192,7 +192,9 @@ int rgf_autotune(rgf_ctx* ctx, int64_t b, int64_t n_energies, int32_t f32, int32
 
 /* Accumulated launch statistics while RGF_OPT_PROFILE is on (synchronises
  * the stream).  Flops are algorithmic: 8*m*n*k real flops per complex
- * product term, 8*n^3 per n x n inversion.  No reference counterpart. */
+ * product term, 8*n^3 per n x n inversion.  gemm_bytes are algorithmic too:
+ * per problem, op(A) and op(B) of each term and each addend read once, D
+ * (and D2) written once, 16 bytes per element.  No reference counterpart. */
 typedef struct rgf_profile {
   int64_t gemm_launches;
   double gemm_ms;
@@ -200,6 +202,7 @@ typedef struct rgf_profile {
   int64_t inv_calls;
   double inv_ms;
   double inv_flops;
+  double gemm_bytes;
 } rgf_profile;
 int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset);
 

@@ -107,6 +107,7 @@ class Profile_t(ctypes.Structure):
         ("inv_calls", c_int64),
         ("inv_ms", c_double),
         ("inv_flops", c_double),
+        ("gemm_bytes", c_double),
     ]
 
 

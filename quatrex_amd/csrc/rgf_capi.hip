@@ -204,13 +204,13 @@ int Ctx::prof_begin(hipEvent_t* a, hipStream_t s) {
   return hip(hipEventRecord(*a, s ? s : stream), "hipEventRecord");
 }
 
-int Ctx::prof_end(hipEvent_t a, double flops, int kind, hipStream_t s) {
+int Ctx::prof_end(hipEvent_t a, double flops, int kind, hipStream_t s, double bytes) {
   if (!profile || !a) return RGF_OK;
   hipEvent_t b = take_event();
   if (!b) return fail(RGF_EHIP, "hipEventCreate");
   int rc = hip(hipEventRecord(b, s ? s : stream), "hipEventRecord");
   if (rc) return rc;
-  timed.push_back(Timed{a, b, flops, kind});
+  timed.push_back(Timed{a, b, flops, bytes, kind});
   if (timed.size() >= 4096) return prof_collect();
   return RGF_OK;
 }
@@ -222,7 +222,12 @@ int Ctx::prof_collect() {
     float ms = 0.f;
     e = hipEventElapsedTime(&ms, t.a, t.b);
     if (e != hipSuccess) return hip(e, "hipEventElapsedTime");
-    if (t.kind == 0) { prof_gemm_launches++; prof_gemm_ms += ms; prof_gemm_flops += t.flops; }
+    if (t.kind == 0) {
+      prof_gemm_launches++;
+      prof_gemm_ms += ms;
+      prof_gemm_flops += t.flops;
+      prof_gemm_bytes += t.bytes;
+    }
     else { prof_inv_calls++; prof_inv_ms += ms; prof_inv_flops += t.flops; }
     event_pool.push_back(t.a);
     event_pool.push_back(t.b);
@@ -237,6 +242,19 @@ static double group_flops(const ZGroup& g) {
     for (int t = 0; t < g.p[p].nterms; ++t)
       f += 8.0 * g.p[p].m * (double)g.p[p].n * g.p[p].t[t].k;
   return f * g.nE;
+}
+
+// Algorithmic HBM bytes of one launch: every problem reads op(A) and op(B)
+// of each term and each addend once and writes D (and D2) once.
+static double group_bytes(const ZGroup& g) {
+  double el = 0;
+  for (int p = 0; p < g.nprob; ++p) {
+    const ZProblem& P = g.p[p];
+    const double mn = (double)P.m * P.n;
+    for (int t = 0; t < P.nterms; ++t) el += (double)P.t[t].k * (P.m + P.n);
+    el += mn * (P.naddends + 1 + (P.d2mode != kD2None ? 1 : 0));
+  }
+  return el * sizeof(double2) * g.nE;
 }
 
 int Ctx::gemm(const std::vector<ZProblem>& probs, int nE, int b) {
@@ -259,7 +277,7 @@ int Ctx::gemm(const std::vector<ZProblem>& probs, int nE, int b) {
     rc = hip(prec32 ? launch_zgemm_f32(g, tile_for(b), stream) : launch_zgemm(g, tile_for(b), stream),
              "zgemm launch");
     if (rc) return rc;
-    rc = prof_end(ev, group_flops(g), 0);
+    rc = prof_end(ev, group_flops(g), 0, nullptr, group_bytes(g));
     if (rc) return rc;
   }
   return RGF_OK;
@@ -1501,12 +1519,14 @@ int rgf_profile_read(rgf_ctx* ctx, rgf_profile* out, int32_t reset) {
   out->gemm_launches = ctx->prof_gemm_launches;
   out->gemm_ms = ctx->prof_gemm_ms;
   out->gemm_flops = ctx->prof_gemm_flops;
+  out->gemm_bytes = ctx->prof_gemm_bytes;
   out->inv_calls = ctx->prof_inv_calls;
   out->inv_ms = ctx->prof_inv_ms;
   out->inv_flops = ctx->prof_inv_flops;
   if (reset) {
     ctx->prof_gemm_launches = ctx->prof_inv_calls = 0;
-    ctx->prof_gemm_ms = ctx->prof_gemm_flops = ctx->prof_inv_ms = ctx->prof_inv_flops = 0;
+    ctx->prof_gemm_ms = ctx->prof_gemm_flops = ctx->prof_gemm_bytes = 0;
+    ctx->prof_inv_ms = ctx->prof_inv_flops = 0;
   }
   return RGF_OK;
 }

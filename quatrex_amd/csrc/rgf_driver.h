@@ -71,14 +71,15 @@ struct Ctx {
 
   // launch profiling (RGF_OPT_PROFILE)
   bool profile = false;
-  struct Timed { hipEvent_t a, b; double flops; int kind; };
+  struct Timed { hipEvent_t a, b; double flops, bytes; int kind; };
   std::vector<Timed> timed;
   std::vector<hipEvent_t> event_pool;
   int64_t prof_gemm_launches = 0, prof_inv_calls = 0;
-  double prof_gemm_ms = 0, prof_gemm_flops = 0, prof_inv_ms = 0, prof_inv_flops = 0;
+  double prof_gemm_ms = 0, prof_gemm_flops = 0, prof_gemm_bytes = 0, prof_inv_ms = 0,
+         prof_inv_flops = 0;
   hipEvent_t take_event();
   int prof_begin(hipEvent_t* a, hipStream_t s = nullptr);
-  int prof_end(hipEvent_t a, double flops, int kind, hipStream_t s = nullptr);
+  int prof_end(hipEvent_t a, double flops, int kind, hipStream_t s = nullptr, double bytes = 0);
   int prof_collect();
 
   virtual ~Ctx();

@@ -56,17 +56,23 @@ inline int panel_nb(int n) { return n <= 512 ? 32 : 16; }
 
 // Shared state of one panel workgroup.
 // Panel rows are moved between HBM and registers through LDS in chunks of
-// kChunk columns, so that global accesses are row-contiguous.
+// kChunk columns, so that global accesses are row-contiguous.  That staging
+// area is dynamic LDS of blockDim.x x (kChunk + 1) elements, sized by the
+// launch (one row per thread): a static one sized for the largest block
+// would cap small matrices at two workgroups per CU.
 constexpr int kChunk = 8;
 
-template <int NB, int MAXT>
+template <int NB>
 struct PanelShared {
   double red_key[16];
   int red_idx[16];
   double2 piv_row[NB];
   int s_piv[NB];
-  double2 stage[MAXT][kChunk + 1];
 };
+
+inline size_t panel_stage_bytes(int threads) {
+  return (size_t)threads * (kChunk + 1) * sizeof(double2);
+}
 
 template <int NB, int MAXT>
 __global__ void __launch_bounds__(MAXT)
@@ -79,7 +85,9 @@ zinv_panel_kernel(double2* S, int64_t sS, int n, int k, int nbe, char* aux, int6
   int* piv = reinterpret_cast<int*>(ax + PanelAux<NB>::piv(n));
   int* step_of = reinterpret_cast<int*>(ax + PanelAux<NB>::step(n));
 
-  __shared__ PanelShared<NB, MAXT> sh;
+  __shared__ PanelShared<NB> sh;
+  extern __shared__ __align__(16) double2 stage_raw[];
+  auto stage = reinterpret_cast<double2(*)[kChunk + 1]>(stage_raw);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
@@ -100,12 +108,12 @@ zinv_panel_kernel(double2* S, int64_t sS, int n, int k, int nbe, char* aux, int6
     for (int i = 0; i < kChunk; ++i) {
       const int idx = tid + i * rows;
       const int rr = idx / kChunk, q = q0 + idx % kChunk;
-      sh.stage[rr][idx % kChunk] = (rr < n && q < nbe) ? A[(int64_t)rr * n + k + q] : cmk(0.0, 0.0);
+      stage[rr][idx % kChunk] = (rr < n && q < nbe) ? A[(int64_t)rr * n + k + q] : cmk(0.0, 0.0);
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
-      const double2 v = sh.stage[r][q];
+      const double2 v = stage[r][q];
       re[q0 + q] = v.x;
       im[q0 + q] = v.y;
     }
@@ -187,14 +195,14 @@ zinv_panel_kernel(double2* S, int64_t sS, int n, int k, int nbe, char* aux, int6
     for (int q = 0; q < NB; ++q) {
       int col = q + nbe;
       col = (col >= NB) ? col - NB : col;
-      if (col >= q0 && col < q0 + kChunk) sh.stage[r][col - q0] = cmk(re[q], im[q]);
+      if (col >= q0 && col < q0 + kChunk) stage[r][col - q0] = cmk(re[q], im[q]);
     }
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
       const int idx = tid + i * rows;
       const int rr = idx / kChunk, q = q0 + idx % kChunk;
-      if (rr < n && q < nbe) A[(int64_t)rr * n + k + q] = sh.stage[rr][idx % kChunk];
+      if (rr < n && q < nbe) A[(int64_t)rr * n + k + q] = stage[rr][idx % kChunk];
     }
     __syncthreads();
   }
@@ -274,8 +282,9 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
   double2* wb = reinterpret_cast<double2*>(ax + PanelAux<NB>::wp(n, 1));
   const int threads = ((n + 63) / 64) * 64;
   auto panel = [&](int k, int nbe, int slot) {
-    hipLaunchKernelGGL((zinv_panel_kernel<NB, MAXT>), dim3(nE), dim3(threads), 0, stream, S, sS, n,
-                       k, nbe, ax, sAux, slot, info, info_tag, e0);
+    hipLaunchKernelGGL((zinv_panel_kernel<NB, MAXT>), dim3(nE), dim3(threads),
+                       panel_stage_bytes(threads), stream, S, sS, n, k, nbe, ax, sAux, slot, info,
+                       info_tag, e0);
   };
   // operands: columns c0.. of S (stride sS) or of a pivot-row buffer (stride sAuxC), ld n
   auto tS = [&](int c0, const double2* B, int k) {

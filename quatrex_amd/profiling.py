@@ -19,6 +19,13 @@ call counts and durations from a rocprofv3 ``--kernel-trace`` database.
 ``python -m quatrex_amd.profiling inv --n N --energies E`` times the batched
 complex inverse (``rgf_inv_z``) of E diagonally dominant N x N matrices and
 checks the first result against numpy.
+
+``python -m quatrex_amd.profiling sweep --n-blocks N --block-size B --energies E``
+runs the FP64 selected solve of the synthetic device (default tiles, no
+autotuner launches) and prints the sweep products' average launch time,
+flops and algorithmic bytes per launch from the library's HIP-event profile,
+so that a kernel trace and HBM counters of the same command can be set
+beside them.
 """
 
 from __future__ import annotations
@@ -162,6 +169,42 @@ def dbstats(path: str) -> None:
         print(f"{name[:78]:78s} {n:6d} {avg / 1e3:10.1f} {t / 1e6:10.2f} {100 * t / tot:6.1f}")
 
 
+def time_sweep(n_blocks: int, b: int, energies: int, reps: int) -> None:
+    import torch
+
+    from quatrex_amd import solve as S
+    from quatrex_amd import workloads as W
+    from quatrex_amd.layout import Geometry
+
+    dev = torch.device("cuda", 0)
+    g = Geometry.uniform(n_blocks, b)
+    model = W.make_device(g, W.DEVICE_SEED_BASE + 2)
+    A = W.system_matrix_torch(model, W.draw_energies(energies), 1e-3, dev)
+    sld = torch.as_tensor(model.sl_diag, device=dev).expand(energies, -1).contiguous()
+    sgd = torch.as_tensor(model.sg_diag, device=dev).expand(energies, -1).contiguous()
+    slu = torch.zeros((energies, g.offdiag_size), dtype=torch.complex128, device=dev)
+    sgu = torch.zeros_like(slu)
+    GR = S.packed_empty(g, energies, dev)
+    GL = S.packed_empty(g, energies, dev, parts=("d", "u"))
+    GG = S.packed_empty(g, energies, dev, parts=("d", "u"))
+    ctx = S.default_context(dev)
+    ctx.profile(True)
+    for r in range(reps + 1):
+        if r == 1:
+            ctx.profile_read(reset=True)  # the first solve is a warm-up
+        S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
+                         ctx=ctx)
+    p = ctx.profile_read(reset=True)
+    ctx.status()
+    n = max(p["gemm_launches"], 1)
+    ms = p["gemm_ms"] / n
+    print(f"sweep {n_blocks}x{b} energies={energies}: {p['gemm_launches']} product launches, "
+          f"{ms * 1e3:.1f} us, {p['gemm_flops'] / n / 1e9:.4f} GFLOP and "
+          f"{p['gemm_bytes'] / n / 1e6:.2f} MB (algorithmic) per launch: "
+          f"{p['gemm_flops'] / p['gemm_ms'] / 1e9:.2f} TFLOP/s, "
+          f"{p['gemm_bytes'] / p['gemm_ms'] / 1e6:.1f} GB/s algorithmic")
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -185,6 +228,11 @@ def main(argv=None) -> None:
     v.add_argument("--n", type=int, default=256)
     v.add_argument("--energies", type=int, default=512)
     v.add_argument("--reps", type=int, default=4)
+    w = sub.add_parser("sweep")
+    w.add_argument("--n-blocks", type=int, default=32)
+    w.add_argument("--block-size", type=int, default=32)
+    w.add_argument("--energies", type=int, default=4096)
+    w.add_argument("--reps", type=int, default=2)
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy_calibration(args.gib, args.reps)
@@ -196,6 +244,8 @@ def main(argv=None) -> None:
         dbstats(args.path)
     elif args.cmd == "inv":
         time_inverse(args.n, args.energies, args.reps)
+    elif args.cmd == "sweep":
+        time_sweep(args.n_blocks, args.block_size, args.energies, args.reps)
     else:
         summarize(args.paths)
 
