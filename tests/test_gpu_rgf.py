@@ -208,7 +208,8 @@ def _check_solve(device, g, A, SL, SG, obc_np=None, with_current=False):
     return errs
 
 
-@pytest.mark.parametrize("nb,b,ne", [(8, 32, 2), (5, 64, 3), (4, 100, 2), (3, 128, 2), (2, 256, 1)])
+@pytest.mark.parametrize("nb,b,ne", [(8, 32, 2), (5, 64, 3), (4, 100, 2), (3, 128, 2), (2, 256, 1),
+                                     (3, 384, 1), (3, 512, 1), (2, 520, 1)])
 def test_generator_workload_vs_dense(device, nb, b, ne):
     g = Geometry.uniform(nb, b)
     _, A, SL, SG = _generator_case(g, ne, W.DEVICE_SEED_BASE + nb)
