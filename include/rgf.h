@@ -186,6 +186,9 @@ int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_e
  * best_cfg; ms_per_cfg (optional, RGF_NUM_TILE_CFGS floats) receives the
  * timings, -1 for configurations that do not apply.  Allocates temporary
  * device buffers.  No reference counterpart (the autotuner of this build). */
+/* Configurations (index: output tile, threads per workgroup): 0 64x64 (256),
+ * 1 32x32 (64), 2 64x32 (128), 3 128x64 (512).  The FP32-compute products
+ * are built for 0, 1 and 3. */
 #define RGF_NUM_TILE_CFGS 4
 int rgf_autotune(rgf_ctx* ctx, int64_t b, int64_t n_energies, int32_t f32, int32_t* best_cfg,
                  float* ms_per_cfg);

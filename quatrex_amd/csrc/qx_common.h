@@ -89,7 +89,9 @@ int tile_m(int cfg);
 int tile_n(int cfg);
 // Configurations the FP32-compute products are built for (the others fall
 // back to 64x64 there).
-inline bool f32_tile_ok(int cfg) { return cfg == kTile64x64 || cfg == kTile32x32; }
+inline bool f32_tile_ok(int cfg) {
+  return cfg == kTile64x64 || cfg == kTile32x32 || cfg == kTile128x64;
+}
 
 // XCD-aware remap: hardware dispatches consecutive workgroups round-robin over
 // the 8 XCDs; give each XCD a contiguous range of logical ids so tiles of one

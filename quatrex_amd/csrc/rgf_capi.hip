@@ -1018,7 +1018,7 @@ int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value) {
         const int b = option - (f32 ? 3000 : 1000);
         if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
         if (f32 && value >= 0 && !qx::f32_tile_ok((int)value))
-          return ctx->fail(RGF_EINVAL, "FP32-compute products support tile configs 0 and 1");
+          return ctx->fail(RGF_EINVAL, "FP32-compute products support tile configs 0, 1 and 3");
         std::vector<int>& t = f32 ? ctx->tile_table32 : ctx->tile_table;
         if ((int)t.size() <= b) t.resize(b + 1, -1);
         t[b] = (int)value;

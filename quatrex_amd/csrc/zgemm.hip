@@ -413,6 +413,7 @@ hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream) {
 hipError_t launch_zgemm_f32(const ZGroup& g, int cfg, hipStream_t stream) {
   switch (cfg) {
     case kTile32x32: return launch_cfg<32, 32, 0, true>(g, stream);
+    case kTile128x64: return launch_cfg<128, 64, 0, true>(g, stream);
     default: return launch_cfg<64, 64, 0, true>(g, stream);
   }
 }

@@ -84,10 +84,10 @@ def test_mixed_error_is_fp32_level(device, eta):
     assert max(errs) > 1e-12, errs  # it really ran in reduced precision
 
 
-@pytest.mark.parametrize("cfg", [0, 1])
+@pytest.mark.parametrize("cfg", [0, 1, 3])
 def test_mixed_every_f32_tile_config(device, cfg):
     """Each tile configuration built for the FP32-compute products (64x64,
-    32x32) gives FP32-level results at b = 200 (partial tiles)."""
+    32x32, 128x64) gives FP32-level results at b = 200 (partial tiles)."""
     g, A, SL, SG, t = _case(device, 4, 200, 2, 1e-2, 3004)
     ctx = S.default_context(device)
     ctx.set_option(_lib.RGF_OPT_TILE_CFG, cfg)
