@@ -309,6 +309,40 @@ def test_energy_alone_equals_in_batch(device):
             assert torch.equal(pb[2:3], pa)
 
 
+def test_profile_counters(device):
+    """The HIP-event profile that bench.py reports: per energy the launched
+    product flops lie within 5% of the reference op count 8 b^3 (33 N - 29)
+    (this build fuses and adds a few products), the algorithmic bytes give the
+    b/6-type intensity of b x b products, inversions count 8 b^3 N, and
+    reading with reset clears the counters."""
+    import torch
+
+    nb, b, E = 6, 64, 3
+    g = Geometry.uniform(nb, b)
+    _, A, SL, SG = _generator_case(g, E, 71)
+    t = lambda x: _t(x, device)  # noqa
+    ctx = S.Context(device)
+    try:
+        GR, GL, GG = (S.packed_empty(g, E, device) for _ in range(3))
+        ctx.profile(True)
+        ctx.profile_read(reset=True)
+        S.selected_solve(g, tuple(map(t, A)), tuple(map(t, SL)), tuple(map(t, SG)), g_lesser=GL,
+                         g_greater=GG, g_retarded=GR, ctx=ctx)
+        p = ctx.profile_read(reset=True)
+        ctx.status()
+        torch.cuda.synchronize()
+        assert p["gemm_launches"] > 0 and p["gemm_ms"] > 0 and p["inv_ms"] > 0
+        ref = 8.0 * b**3 * (33 * nb - 29) * E
+        assert abs(p["gemm_flops"] - ref) / ref < 0.05, (p["gemm_flops"], ref)
+        assert p["inv_calls"] == nb and p["inv_flops"] == 8.0 * b**3 * nb * E
+        intensity = p["gemm_flops"] / p["gemm_bytes"]
+        assert b / 12 < intensity < b / 4, intensity
+        again = ctx.profile_read(reset=True)
+        assert again["gemm_launches"] == 0 and again["gemm_bytes"] == 0 and again["inv_calls"] == 0
+    finally:
+        ctx.close()
+
+
 def test_caller_workspace_solve_exact_and_chunked(device):
     """A caller-owned workspace of exactly rgf_workspace_size bytes for all
     energies, or for 2 of the 5 (the sweep then runs in chunks), gives a
