@@ -18,30 +18,38 @@ namespace qx {
 
 namespace {
 
-constexpr int TK = 8;
 constexpr int PAD = 1;
-constexpr int kSlab = 32;  // epilogue rows per LDS pass: the rows of one wave row
 
-template <int TM, int TN>
+// kTK: k-depth of one LDS stage.  kSlab: epilogue rows per LDS pass (the rows
+// of one wave row).  The one-wave 32x32 tile (blocks below 64, where the
+// products are bound by HBM) halves both for FP64, so that its LDS (8.4 KB
+// instead of 16.9 KB) does not cap the number of workgroups per CU below what
+// its registers allow (measured: equal at b = 32, +13-34% at b = 48); FP32
+// products, whose MFMAs take half as long, lose more to the extra barriers
+// than they gain and keep the deeper stage.
+template <int TM, int TN, bool kF32 = false>
 struct TileShape {
+  static constexpr bool kSmall = TM == 32 && TN == 32 && !kF32;
+  static constexpr int kTK = kSmall ? 4 : 8;
+  static constexpr int kSlab = kSmall ? 16 : 32;
   static constexpr int kWavesM = TM / 32;
   static constexpr int kWavesN = TN / 32;
   static constexpr int kThreads = 64 * kWavesM * kWavesN;
-  static constexpr int kAPer = TM * TK / kThreads;
-  static constexpr int kBPer = TK * TN / kThreads;
+  static constexpr int kAPer = TM * kTK / kThreads;
+  static constexpr int kBPer = kTK * TN / kThreads;
 };
 
-template <int TM, int TN>
+template <int TM, int TN, bool F32>
 __device__ inline void load_a(const ZTerm& t, int e, int m, int m0, int k0, int tid,
-                              double2 (&ra)[TileShape<TM, TN>::kAPer]) {
-  using S = TileShape<TM, TN>;
+                              double2 (&ra)[TileShape<TM, TN, F32>::kAPer]) {
+  using S = TileShape<TM, TN, F32>;
   const double2* A = t.A + (int64_t)e * t.sA;
   const double2 zero = cmk(0.0, 0.0);
 #pragma unroll
   for (int i = 0; i < S::kAPer; ++i) {
     const int idx = tid + i * S::kThreads;
     int mm, kk;
-    if (t.opA == kOpN) { mm = idx / TK; kk = idx % TK; }
+    if (t.opA == kOpN) { mm = idx / S::kTK; kk = idx % S::kTK; }
     else { mm = idx % TM; kk = idx / TM; }
     const int gm = m0 + mm, gk = k0 + kk;
     double2 v = zero;
@@ -55,16 +63,17 @@ __device__ inline void load_a(const ZTerm& t, int e, int m, int m0, int k0, int 
 __device__ inline void put(double2* p, double2 v) { *p = v; }
 __device__ inline void put(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
 
-template <int TM, int TN, typename LT>
-__device__ inline void store_a(const ZTerm& t, int tid, const double2 (&ra)[TileShape<TM, TN>::kAPer],
+template <int TM, int TN, bool F32, typename LT>
+__device__ inline void store_a(const ZTerm& t, int tid,
+                               const double2 (&ra)[TileShape<TM, TN, F32>::kAPer],
                                LT (*As)[TM + PAD]) {
-  using S = TileShape<TM, TN>;
+  using S = TileShape<TM, TN, F32>;
   const bool cj = (t.opA == kOpC);
 #pragma unroll
   for (int i = 0; i < S::kAPer; ++i) {
     const int idx = tid + i * S::kThreads;
     int mm, kk;
-    if (!cj) { mm = idx / TK; kk = idx % TK; }
+    if (!cj) { mm = idx / S::kTK; kk = idx % S::kTK; }
     else { mm = idx % TM; kk = idx / TM; }
     double2 v = ra[i];
     if (cj) v.y = -v.y;
@@ -72,10 +81,10 @@ __device__ inline void store_a(const ZTerm& t, int tid, const double2 (&ra)[Tile
   }
 }
 
-template <int TM, int TN>
+template <int TM, int TN, bool F32>
 __device__ inline void load_b(const ZTerm& t, int e, int n, int n0, int k0, int tid,
-                              double2 (&rb)[TileShape<TM, TN>::kBPer]) {
-  using S = TileShape<TM, TN>;
+                              double2 (&rb)[TileShape<TM, TN, F32>::kBPer]) {
+  using S = TileShape<TM, TN, F32>;
   const double2* B = t.B + (int64_t)e * t.sB;
   const double2 zero = cmk(0.0, 0.0);
 #pragma unroll
@@ -83,7 +92,7 @@ __device__ inline void load_b(const ZTerm& t, int e, int n, int n0, int k0, int 
     const int idx = tid + i * S::kThreads;
     int kk, nn;
     if (t.opB == kOpN) { kk = idx / TN; nn = idx % TN; }
-    else { kk = idx % TK; nn = idx / TK; }
+    else { kk = idx % S::kTK; nn = idx / S::kTK; }
     const int gk = k0 + kk, gn = n0 + nn;
     double2 v = zero;
     if (gn < n && gk < t.k) {
@@ -93,17 +102,18 @@ __device__ inline void load_b(const ZTerm& t, int e, int n, int n0, int k0, int 
   }
 }
 
-template <int TM, int TN, typename LT>
-__device__ inline void store_b(const ZTerm& t, int tid, const double2 (&rb)[TileShape<TM, TN>::kBPer],
+template <int TM, int TN, bool F32, typename LT>
+__device__ inline void store_b(const ZTerm& t, int tid,
+                               const double2 (&rb)[TileShape<TM, TN, F32>::kBPer],
                                LT (*Bs)[TN + PAD]) {
-  using S = TileShape<TM, TN>;
+  using S = TileShape<TM, TN, F32>;
   const bool cj = (t.opB == kOpC);
 #pragma unroll
   for (int i = 0; i < S::kBPer; ++i) {
     const int idx = tid + i * S::kThreads;
     int kk, nn;
     if (!cj) { kk = idx / TN; nn = idx % TN; }
-    else { kk = idx % TK; nn = idx / TK; }
+    else { kk = idx % S::kTK; nn = idx / S::kTK; }
     double2 v = rb[i];
     if (cj) v.y = -v.y;
     put(&Bs[kk][nn], v);
@@ -112,11 +122,12 @@ __device__ inline void store_b(const ZTerm& t, int tid, const double2 (&rb)[Tile
 
 template <int TM, int TN, typename LT>
 struct Smem {
-  static constexpr int kA = TK * (TM + PAD);  // one LDS stage of op(A), k-major (LT elements)
-  static constexpr int kB = TK * (TN + PAD);  // one LDS stage of op(B), k-major
+  using S = TileShape<TM, TN, std::is_same<LT, float2>::value>;
+  static constexpr int kA = S::kTK * (TM + PAD);  // one LDS stage of op(A), k-major (LT elements)
+  static constexpr int kB = S::kTK * (TN + PAD);  // one LDS stage of op(B), k-major
   static constexpr int kMainBytes = (2 * kA + 2 * kB) * (int)sizeof(LT);
   // the epilogue stages one kSlab-row slab of the output tile at a time
-  static constexpr int kTileBytes = kSlab * (TN + 1) * (int)sizeof(double2);
+  static constexpr int kTileBytes = S::kSlab * (TN + 1) * (int)sizeof(double2);
   static constexpr int kBytes = kMainBytes > kTileBytes ? kMainBytes : kTileBytes;
 };
 
@@ -136,13 +147,15 @@ __device__ inline double4_t mfma_nega(double a, double b, double4_t c) {
 // Occupancy target: four 64x64 workgroups per CU (33 KB LDS each, <= 128
 // VGPRs), measured faster than three.
 template <int TM, int TN, int kRole, bool kF32>
-__global__ void __launch_bounds__((TileShape<TM, TN>::kThreads),
+__global__ void __launch_bounds__((TileShape<TM, TN, kF32>::kThreads),
                                   (TM * TN > 4096 ? 1 : (TM * TN == 4096 ? 4 : 2)))
 zgemm_grouped_kernel(const ZGroup g) {
-  using S = TileShape<TM, TN>;
+  using S = TileShape<TM, TN, kF32>;
   using LT = typename std::conditional<kF32, float2, double2>::type;
   using AccT = typename std::conditional<kF32, float4_t, double4_t>::type;
   using L = Smem<TM, TN, LT>;
+  constexpr int TK = S::kTK;
+  constexpr int kSlab = S::kSlab;
   __shared__ __align__(16) char smem_raw[L::kBytes];
   LT* smem = reinterpret_cast<LT*>(smem_raw);
   auto As = [&](int buf) { return reinterpret_cast<LT(*)[TM + PAD]>(smem + buf * L::kA); };
@@ -187,17 +200,17 @@ zgemm_grouped_kernel(const ZGroup g) {
     const ZTerm T = P.t[ti];
     const int nk = (T.k + TK - 1) / TK;
     double2 ra[S::kAPer], rb[S::kBPer];
-    load_a<TM, TN>(T, e, m, m0, 0, tid, ra);
-    load_b<TM, TN>(T, e, n, n0, 0, tid, rb);
-    store_a<TM, TN, LT>(T, tid, ra, As(0));
-    store_b<TM, TN, LT>(T, tid, rb, Bs(0));
+    load_a<TM, TN, kF32>(T, e, m, m0, 0, tid, ra);
+    load_b<TM, TN, kF32>(T, e, n, n0, 0, tid, rb);
+    store_a<TM, TN, kF32, LT>(T, tid, ra, As(0));
+    store_b<TM, TN, kF32, LT>(T, tid, rb, Bs(0));
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
       const int cur = kt & 1;
       const bool more = (kt + 1 < nk);
       if (more) {
-        load_a<TM, TN>(T, e, m, m0, (kt + 1) * TK, tid, ra);
-        load_b<TM, TN>(T, e, n, n0, (kt + 1) * TK, tid, rb);
+        load_a<TM, TN, kF32>(T, e, m, m0, (kt + 1) * TK, tid, ra);
+        load_b<TM, TN, kF32>(T, e, n, n0, (kt + 1) * TK, tid, rb);
       }
       auto as = As(cur);
       auto bs = Bs(cur);
@@ -261,8 +274,8 @@ zgemm_grouped_kernel(const ZGroup g) {
         }
       }
       if (more) {
-        store_a<TM, TN, LT>(T, tid, ra, As(cur ^ 1));
-        store_b<TM, TN, LT>(T, tid, rb, Bs(cur ^ 1));
+        store_a<TM, TN, kF32, LT>(T, tid, ra, As(cur ^ 1));
+        store_b<TM, TN, kF32, LT>(T, tid, rb, Bs(cur ^ 1));
       }
       __syncthreads();
     }
@@ -279,20 +292,25 @@ zgemm_grouped_kernel(const ZGroup g) {
   const int64_t eo = (int64_t)e;
   double2* D = P.D + eo * P.sD;
   double2* D2 = P.D2 ? P.D2 + eo * P.sD2 : nullptr;
+  constexpr int kSlabsPerWaveRow = 32 / kSlab;  // 1, or 2 for the 32x32 tile
 #pragma unroll 1
-  for (int h = 0; h < S::kWavesM; ++h) {
+  for (int h = 0; h < TM / kSlab; ++h) {
     const int r0 = m0 + h * kSlab;  // first global row of the slab
-    if (wm == h) {
+    if (wm == h / kSlabsPerWaveRow) {
+      const int isub = h % kSlabsPerWaveRow;  // sub-tile row held by a half-row slab
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i) {
+        if (kSlabsPerWaveRow == 2 && i != isub) continue;
+        const int ri = kSlabsPerWaveRow == 1 ? i * 16 : 0;
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int rr = kF32 ? ((lane >> 4) * 4 + r) : (r * 4 + (lane >> 4));
-            Cs[i * 16 + rr][wn * 32 + j * 16 + (lane & 15)] =
+            Cs[ri + rr][wn * 32 + j * 16 + (lane & 15)] =
                 cmk((double)accr[i][j][r], (double)acci[i][j][r]);
           }
+      }
     }
     __syncthreads();
     // Each thread owns elements idx = tid + q*kThreads of the slab (row-major,
@@ -376,7 +394,7 @@ hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   if (total == 0) return hipSuccess;
   if (total > 0x7fffffff) return hipErrorInvalidValue;
   hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole, kF32>), dim3((unsigned)total),
-                     dim3(TileShape<TM, TN>::kThreads), 0, stream, g);
+                     dim3(TileShape<TM, TN, kF32>::kThreads), 0, stream, g);
   return hipGetLastError();
 }
 
