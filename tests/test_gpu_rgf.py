@@ -276,6 +276,62 @@ def test_every_tile_config_vs_dense(device, cfg):
     assert max(errs) <= TOL, errs
 
 
+def _edge_case(bs, seed, E=2):
+    rng = np.random.default_rng(seed)
+    g = Geometry(tuple(bs))
+    n = g.n
+    a = np.empty((E, n, n), np.complex128)
+    for e in range(E):
+        h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        a[e] = (0.3 * e + 4.0 + 1e-2j) * np.eye(n) - 0.25 * (h + h.conj().T) * tridiagonal_mask(bs)
+    A = g.from_dense(a)
+    sl = np.broadcast_to(skew_tridiagonal(bs, rng), (E, n, n))
+    sg = np.broadcast_to(skew_tridiagonal(bs, rng), (E, n, n))
+
+    def one(b, f):
+        br = -0.5j * np.eye(b) + 0.05 * (rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b)))
+        gam = 1j * (br - br.conj().T)
+        rep = lambda x: np.broadcast_to(x, (E, b, b)).copy()  # noqa
+        return rep(br), rep(1j * gam * f), rep(1j * gam * (f - 1))
+
+    N = len(bs)
+    L, R = one(bs[0], 0.6), one(bs[-1], 0.1)
+    if N == 1:  # both contacts attach to the one block
+        obc = [[L[k] + R[k]] for k in range(3)]
+    else:
+        obc = [[L[k]] + [None] * (N - 2) + [R[k]] for k in range(3)]
+    return g, A, g.from_dense(sl)[:2], g.from_dense(sg)[:2], obc
+
+
+@pytest.mark.parametrize("bs", [[40], [1] * 8, [1, 3, 1, 2], [33]])
+def test_edge_geometries_vs_dense(device, bs):
+    """A single block (the sweeps are empty; both boundary terms of the
+    current read block 0) and 1 x 1 blocks, with off-diagonal Sigma, both
+    contacts and the current, against the dense oracle."""
+    g, A, SL, SG, obc = _edge_case(bs, 500 + len(bs))
+    errs = _check_solve(device, g, A, SL, SG, obc_np=obc, with_current=True)
+    assert max(errs) <= TOL, errs
+
+
+def test_zero_energies_are_a_no_op(device):
+    """n_energies = 0: both entry points return without launching work and
+    leave the (empty) outputs alone."""
+    import torch
+
+    g = Geometry((8, 8, 8))
+    z = lambda shape: torch.zeros(shape, dtype=torch.complex128, device=device)  # noqa
+    A = (z((0, g.diag_size)), z((0, g.offdiag_size)), z((0, g.offdiag_size)))
+    SL = (z((0, g.diag_size)), z((0, g.offdiag_size)))
+    SG = (z((0, g.diag_size)), z((0, g.offdiag_size)))
+    GR, GL, GG = (S.packed_empty(g, 0, device) for _ in range(3))
+    S.selected_solve(g, A, SL, SG, g_lesser=GL, g_greater=GG, g_retarded=GR)
+    X = S.packed_empty(g, 0, device)
+    S.selected_inv(g, A, x=X)
+    S.default_context(device).status()
+    torch.cuda.synchronize()
+    assert GR[0].shape[0] == 0 and X[0].shape[0] == 0
+
+
 def test_retarded_obc_equals_folded_boundary(device):
     """Sigma_B folded into A, or passed as OBCBlocks.retarded, gives the same result."""
     g = Geometry.uniform(6, 48)
