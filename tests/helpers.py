@@ -69,3 +69,36 @@ def skew_tridiagonal(block_sizes, rng, scale=0.1):
     x = scale * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
     x = x * tridiagonal_mask(block_sizes)
     return x - x.conj().T
+
+
+def edge_case(bs, seed, E=2):
+    """(geometry, A, Sigma^<, Sigma^>, OBC lists) for edge geometries (one block,
+    1 x 1 blocks): a Hermitian tridiagonal H shifted off the real axis,
+    skew-Hermitian Sigma with off-diagonal blocks, retarded / lesser / greater
+    boundary blocks of both contacts (summed when there is one block)."""
+    from quatrex_amd.layout import Geometry
+
+    rng = np.random.default_rng(seed)
+    g = Geometry(tuple(bs))
+    n = g.n
+    a = np.empty((E, n, n), np.complex128)
+    for e in range(E):
+        h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        a[e] = (0.3 * e + 4.0 + 1e-2j) * np.eye(n) - 0.25 * (h + h.conj().T) * tridiagonal_mask(bs)
+    A = g.from_dense(a)
+    sl = np.broadcast_to(skew_tridiagonal(bs, rng), (E, n, n))
+    sg = np.broadcast_to(skew_tridiagonal(bs, rng), (E, n, n))
+
+    def one(b, f):
+        br = -0.5j * np.eye(b) + 0.05 * (rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b)))
+        gam = 1j * (br - br.conj().T)
+        rep = lambda x: np.broadcast_to(x, (E, b, b)).copy()  # noqa
+        return rep(br), rep(1j * gam * f), rep(1j * gam * (f - 1))
+
+    N = len(bs)
+    L, R = one(bs[0], 0.6), one(bs[-1], 0.1)
+    if N == 1:  # both contacts attach to the one block
+        obc = [[L[k] + R[k]] for k in range(3)]
+    else:
+        obc = [[L[k]] + [None] * (N - 2) + [R[k]] for k in range(3)]
+    return g, A, g.from_dense(sl)[:2], g.from_dense(sg)[:2], obc

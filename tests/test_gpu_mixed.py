@@ -9,7 +9,7 @@ from quatrex_amd import _lib
 from quatrex_amd import solve as S
 from quatrex_amd import workloads as W
 from quatrex_amd.layout import Geometry
-from tests.helpers import block_rel_errors, tridiagonal_mask
+from tests.helpers import block_rel_errors, edge_case, tridiagonal_mask
 
 pytestmark = pytest.mark.gpu
 
@@ -98,6 +98,54 @@ def test_mixed_every_f32_tile_config(device, cfg):
     errs = _errors(g, outs, A, SL, SG)
     assert max(errs) < 1e-3, errs
     assert max(errs) > 1e-12, errs
+
+
+@pytest.mark.parametrize("bs", [[40], [1] * 8])
+def test_mixed_edge_geometries(device, bs):
+    """The mixed-precision sweep on a single block and on 1 x 1 blocks, with
+    both contacts: FP32-level agreement with the dense FP64 oracle; with the
+    FP64 check at 1e-8, every energy either fell back (then <= 1e-10 per
+    block) or reports a residual <= 1e-8."""
+    import torch
+
+    from oracle.dense import dense_solve
+
+    g, A, SL, SG, obc = edge_case(bs, 600 + len(bs))
+    t = lambda x: torch.as_tensor(np.ascontiguousarray(x), device=device)  # noqa: E731
+    obc_t = [[t(x) if x is not None else None for x in lst] for lst in obc]
+    E = A[0].shape[0]
+
+    def lower(u):
+        out = np.empty_like(u)
+        for i in range(g.n_blocks - 1):
+            g.lower(out, i)[...] = -np.conj(np.swapaxes(g.upper(u, i), -1, -2))
+        return out
+
+    ref = dense_solve(g.to_dense(*A), g.to_dense(SL[0], SL[1], lower(SL[1])),
+                      g.to_dense(SG[0], SG[1], lower(SG[1])), g.block_offsets, obc)
+    m = tridiagonal_mask(list(g.block_sizes))
+    bs_ = list(g.block_sizes)
+    for tol in (0.0, 1e-8):
+        GR, GL, GG = (S.packed_empty(g, E, device) for _ in range(3))
+        res = torch.zeros(E, dtype=torch.float64, device=device)
+        flags = torch.zeros(E, dtype=torch.int32, device=device)
+        S.selected_solve(g, tuple(map(t, A)), tuple(map(t, SL)), tuple(map(t, SG)), g_lesser=GL,
+                         g_greater=GG, g_retarded=GR, obc=obc_t, precision="mixed",
+                         mixed=S.MixedOptions(refine=1, fallback_tol=tol, residuals=res,
+                                              fallback_flags=flags))
+        S.default_context(device).status()
+        outs = [g.to_dense(*(x.cpu().numpy() for x in X)) for X in (GR, GL, GG)]
+        if tol == 0.0:
+            errs = [block_rel_errors(o, r * m, bs_) for o, r in zip(outs, ref)]
+            assert max(errs) < 1e-3, errs
+            continue
+        fl, rs = flags.cpu().numpy(), res.cpu().numpy()
+        for e in range(E):
+            errs = [block_rel_errors(o[e:e + 1], (r * m)[e:e + 1], bs_) for o, r in zip(outs, ref)]
+            if fl[e]:
+                assert max(errs) <= 1e-10, (e, errs)
+            else:
+                assert rs[e] <= 1e-8 and max(errs) < 1e-3, (e, rs[e], errs)
 
 
 def test_mixed_residual_matches_cpu_and_flags_fallback(device):

@@ -19,7 +19,7 @@ from quatrex_amd.datastructures import BlockTridiagonal
 from quatrex_amd.greens_function_solver import RGF, OBCBlocks
 from quatrex_amd.layout import Geometry
 from tests.helpers import (BATCHING, BLOCK_SIZES, STACK_SHAPES, block_rel_errors, bt_dense,
-                           skew_tridiagonal, tridiagonal_mask)
+                           edge_case, skew_tridiagonal, tridiagonal_mask)
 
 pytestmark = pytest.mark.gpu
 
@@ -276,39 +276,12 @@ def test_every_tile_config_vs_dense(device, cfg):
     assert max(errs) <= TOL, errs
 
 
-def _edge_case(bs, seed, E=2):
-    rng = np.random.default_rng(seed)
-    g = Geometry(tuple(bs))
-    n = g.n
-    a = np.empty((E, n, n), np.complex128)
-    for e in range(E):
-        h = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
-        a[e] = (0.3 * e + 4.0 + 1e-2j) * np.eye(n) - 0.25 * (h + h.conj().T) * tridiagonal_mask(bs)
-    A = g.from_dense(a)
-    sl = np.broadcast_to(skew_tridiagonal(bs, rng), (E, n, n))
-    sg = np.broadcast_to(skew_tridiagonal(bs, rng), (E, n, n))
-
-    def one(b, f):
-        br = -0.5j * np.eye(b) + 0.05 * (rng.standard_normal((b, b)) + 1j * rng.standard_normal((b, b)))
-        gam = 1j * (br - br.conj().T)
-        rep = lambda x: np.broadcast_to(x, (E, b, b)).copy()  # noqa
-        return rep(br), rep(1j * gam * f), rep(1j * gam * (f - 1))
-
-    N = len(bs)
-    L, R = one(bs[0], 0.6), one(bs[-1], 0.1)
-    if N == 1:  # both contacts attach to the one block
-        obc = [[L[k] + R[k]] for k in range(3)]
-    else:
-        obc = [[L[k]] + [None] * (N - 2) + [R[k]] for k in range(3)]
-    return g, A, g.from_dense(sl)[:2], g.from_dense(sg)[:2], obc
-
-
 @pytest.mark.parametrize("bs", [[40], [1] * 8, [1, 3, 1, 2], [33]])
 def test_edge_geometries_vs_dense(device, bs):
     """A single block (the sweeps are empty; both boundary terms of the
     current read block 0) and 1 x 1 blocks, with off-diagonal Sigma, both
     contacts and the current, against the dense oracle."""
-    g, A, SL, SG, obc = _edge_case(bs, 500 + len(bs))
+    g, A, SL, SG, obc = edge_case(bs, 500 + len(bs))
     errs = _check_solve(device, g, A, SL, SG, obc_np=obc, with_current=True)
     assert max(errs) <= TOL, errs
 
