@@ -25,11 +25,12 @@ def _resident(g, seed, device):
     return m, (t(m.h_diag), t(m.h_upper), t(m.h_lower)), sb
 
 
+@pytest.mark.parametrize("blocks", [(48, 40, 48, 32, 48), (40,)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_graph_replays_match_eager(device, precision):
+def test_graph_replays_match_eager(device, precision, blocks):
     import torch
 
-    g = Geometry((48, 40, 48, 32, 48))
+    g = Geometry(blocks)
     E = 3
     m, h, sb = _resident(g, 77, device)
     z = torch.zeros(E, dtype=torch.complex128, device=device)
