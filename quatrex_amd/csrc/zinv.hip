@@ -6,17 +6,18 @@
 // first physical row on ties).  Nothing is swapped: after all steps the matrix
 // holds M with A^{-1}[i][j] = M[pi(i)][pi^{-1}(j)].  A panel of NB columns
 // [k, k+nb) is eliminated by zinv_panel_kernel (one workgroup per energy, one
-// physical row per thread, the row's panel part in registers): unblocked GJE
-// on the panel with the pivot search by wavefront shuffle max-reduction.  It
+// physical row per thread, the row's panel part in registers; for matrices up
+// to 32 rows zinv_panel2_kernel, two threads per row): unblocked GJE on the
+// panel with the pivot search by wavefront shuffle max-reduction.  It
 // writes the transformed panel T in place, moves the pivot rows' non-panel
 // columns to Wp (nb x n) and zeroes them in S.  The blocked GJE update is
 //   S[:, nonpanel] += T * Wp[:, nonpanel],
 // i.e. S += (T - J) Wp with J[pi(k+q)][q] = 1: J Wp restores the pivot rows,
 // which the move to Wp left at zero.
 //
-// The update reads and writes all of S and bounds the inversion by HBM, so
-// panels are taken in pairs (P1, P2) and the columns R outside both are
-// updated once, with K = nb1 + nb2:
+// Each update reads and writes all of S (with short K it waits on memory more
+// than on the matrix cores), so panels are taken in pairs (P1, P2) and the
+// columns R outside both are updated once, with K = nb1 + nb2:
 //   1. panel P1 -> T1, WA
 //   2. S[:, P2] += T1 WA[:, P2]                   (P2 columns only)
 //   3. panel P2 -> T2, WB; the move zeroes rows pi(P2) of T1 as well, and
