@@ -948,6 +948,25 @@ def main():
         ctx.status()
         eps_mixed = E * args.steps / tm
         errs = {k: rel_err(X, R) for k, X, R in zip(("G^R", "G^<", "G^>"), (GR, GL, GG), ref)}
+        # the two ways of keeping the inversion input accurate, unchecked: one
+        # FP64 Newton step per inverted block (refine=1), or the Schur
+        # complements themselves in FP64 (refine=-1)
+        refined = {}
+        for name, r in (("newton_step", 1), ("fp64_schur", -1)):
+            mo = S.MixedOptions(refine=r, fallback_tol=0.0)
+            S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
+                             ctx=ctx, precision="mixed", mixed=mo)
+            ctx.status()
+            torch.cuda.synchronize()
+            tr0 = time.perf_counter()
+            S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
+                             ctx=ctx, precision="mixed", mixed=mo)
+            torch.cuda.synchronize()
+            tr = time.perf_counter() - tr0
+            ctx.status()
+            refined[name] = {"refine": r, "value": round(E / tr, 3),
+                             "max_rel_err_vs_fp64": {k: float(f"{rel_err(X, R):.3e}") for k, X, R in
+                                                     zip(("G^R", "G^<", "G^>"), (GR, GL, GG), ref)}}
         # mixed + FP64 a-posteriori check, FP64 fallback above 1e-8
         flags = torch.zeros(E, dtype=torch.int32, device=dev)
         resid = torch.zeros(E, dtype=torch.float64, device=dev)
@@ -971,6 +990,7 @@ def main():
             "max_rel_err_vs_fp64": {k: float(f"{v:.3e}") for k, v in errs.items()},
             "error_sample_energies": ns,
             "roofline_frac": round(t_roof * eps_mixed, 4),
+            "refined": refined,
             "with_check_and_fp64_fallback": {
                 "value": round(E / tc, 3),
                 "fallback_fraction": round(float(flags.float().mean()), 4),

@@ -91,7 +91,9 @@ typedef struct rgf_obc {
 
 /* Mixed-precision options for rgf_selected_solve_mixed. */
 typedef struct rgf_mixed_opts {
-  int32_t refine;            /* FP64 residual refinement steps per inverted block (>= 0) */
+  int32_t refine;            /* >= 0: FP64 residual refinement steps per inverted block;
+                                -1: the Schur complement S_j = A_jj - A_ji g_i A_ij (and its
+                                factor A_ji g_i) in FP64, no refinement step */
   double fallback_tol;       /* FP64 re-solve when an energy's refinement residual exceeds this
                                 (<= 0 disables the fallback) */
   int32_t* fallback_flags;   /* optional device int32[n_energies]: 1 where the FP64 fallback ran */
@@ -170,7 +172,9 @@ int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energ
  * rgf_selected_solve_z; the Schur-complement GEMMs and G Sigma G^H triple
  * products run in complex64 on FP32 matrix cores with FP32 accumulation,
  * the diagonal-block inversions in FP64, followed by `opts->refine` FP64
- * residual-refinement steps per inverted block; energies whose refinement
+ * residual-refinement steps per inverted block (or, with refine = -1, the
+ * Schur complement that feeds each inversion computed in FP64, which is
+ * what those steps converge to, for fewer products); energies whose refinement
  * residual exceeds opts->fallback_tol are re-solved in FP64.  No reference
  * counterpart: the precision mode of this build's serving path. */
 int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
@@ -291,7 +295,8 @@ int rgf_boundary_system_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, double et
 /* HIP graph of a complete selected-solve sweep for fixed buffers.
  * rgf_graph_capture_solve records every launch of one sweep over n_energies
  * energies (precision 0: FP64; 1: FP32-compute products with FP64
- * inversions and `refine` Newton steps, without the a-posteriori check) into
+ * inversions and `refine` Newton steps, or FP64 Schur complements with refine
+ * = -1, without the a-posteriori check) into
  * a graph that owns its geometry tables and its workspace - or, for FP64,
  * uses the context's caller-provided workspace (rgf_set_workspace) when that
  * is large enough, which must then outlive the graph; rgf_graph_launch

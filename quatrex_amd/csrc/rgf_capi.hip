@@ -547,23 +547,36 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
     const int bi = (int)g.bs[i], bj = (int)g.bs[j];
     Mat S = sl(0, bj), P = sl(1, bi), Ql = sl(2, bj), Qg = sl(3, bj), Yl = sl(4, bi),
         Yg = sl(5, bi), Wl = sl(6, bj), Wg = sl(7, bj);
-    // P = A_ji g_i ; Y = A_ji g^<>_i
-    rc = c->gemm({PB(bj, bi).term(A.L(i), kOpN, GR.D(i), kOpN, bi).out(P).done(),
-                  PB(bj, bi).term(A.L(i), kOpN, GL.D(i), kOpN, bi).out(Yl).done(),
-                  PB(bj, bi).term(A.L(i), kOpN, GG.D(i), kOpN, bi).out(Yg).done()},
-                 E, bj);
-    if (rc) return rc;
-    // S = A_jj - B^R_j - P A_ij ; Q = P Sigma_ij
-    rc = c->gemm({PB(bj, bj)
-                      .term(P, kOpN, A.U(i), kOpN, bi, MONE)
-                      .add(A.D(j), ONE)
-                      .add(OR(j), MONE)
-                      .out(S)
-                      .done(),
-                  PB(bj, bj).term(P, kOpN, SL.U(i), kOpN, bi).out(Ql).done(),
-                  PB(bj, bj).term(P, kOpN, SG.U(i), kOpN, bi).out(Qg).done()},
-                 E, bj);
-    if (rc) return rc;
+    const ZProblem pP = PB(bj, bi).term(A.L(i), kOpN, GR.D(i), kOpN, bi).out(P).done();
+    const ZProblem pYl = PB(bj, bi).term(A.L(i), kOpN, GL.D(i), kOpN, bi).out(Yl).done();
+    const ZProblem pYg = PB(bj, bi).term(A.L(i), kOpN, GG.D(i), kOpN, bi).out(Yg).done();
+    const ZProblem pS = PB(bj, bj)
+                            .term(P, kOpN, A.U(i), kOpN, bi, MONE)
+                            .add(A.D(j), ONE)
+                            .add(OR(j), MONE)
+                            .out(S)
+                            .done();
+    const ZProblem pQl = PB(bj, bj).term(P, kOpN, SL.U(i), kOpN, bi).out(Ql).done();
+    const ZProblem pQg = PB(bj, bj).term(P, kOpN, SG.U(i), kOpN, bi).out(Qg).done();
+    if (c->prec32 && c->refine < 0) {
+      // Mixed precision with FP64 Schur complements: P and S in FP64, so the
+      // inversion gets the exact FP64 S (the Newton steps' fixed point);
+      // Y and Q stay FP32 products, in one launch after P.
+      c->prec32 = false;
+      rc = c->gemm({pP}, E, bj);
+      if (!rc) rc = c->gemm({pS}, E, bj);
+      c->prec32 = true;
+      if (rc) return rc;
+      rc = c->gemm({pYl, pYg, pQl, pQg}, E, bj);
+      if (rc) return rc;
+    } else {
+      // P = A_ji g_i ; Y = A_ji g^<>_i
+      rc = c->gemm({pP, pYl, pYg}, E, bj);
+      if (rc) return rc;
+      // S = A_jj - B^R_j - P A_ij ; Q = P Sigma_ij
+      rc = c->gemm({pS, pQl, pQg}, E, bj);
+      if (rc) return rc;
+    }
     // g_j = S^{-1} is independent of W: with overlap it runs on the side stream
     const bool ovl = c->overlap;
     if (ovl) {
@@ -1331,6 +1344,7 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
   if (rc) return rc;
   if (n_energies < 1 || n_energies > 65535) return ctx->fail(RGF_EINVAL, "graph: 1 <= n_energies <= 65535");
   if (precision != 0 && precision != 1) return ctx->fail(RGF_EINVAL, "graph: precision 0 or 1");
+  if (refine < -1) return ctx->fail(RGF_EINVAL, "graph: refine must be >= -1");
   const bool multi = g.N > 1;
   if (!a || !a->diag || (multi && (!a->upper || !a->lower)) || !sigma_lesser || !sigma_greater ||
       !sigma_lesser->diag || !sigma_greater->diag ||
@@ -1542,7 +1556,7 @@ int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_e
   int rc = g.init(geom, &ctx->last_error);
   if (rc) return rc;
   if (!opts) return ctx->fail(RGF_EINVAL, "mixed: options required");
-  if (opts->refine < 0) return ctx->fail(RGF_EINVAL, "mixed: refine must be >= 0");
+  if (opts->refine < -1) return ctx->fail(RGF_EINVAL, "mixed: refine must be >= -1");
   if (n_energies < 0) return ctx->fail(RGF_EINVAL, "n_energies < 0");
   if (n_energies == 0) return RGF_OK;
   const bool multi = g.N > 1;

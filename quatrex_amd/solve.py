@@ -203,7 +203,7 @@ def _obc_struct(lists):
 
 @dataclass
 class MixedOptions:
-    refine: int = 1
+    refine: int = 1  # FP64 refinement steps; -1: FP64 Schur complements instead
     fallback_tol: float = 1e-8
     fallback_flags: object = None  # device int32 (E,)
     residuals: object = None  # device float64 (E,)

@@ -26,8 +26,8 @@ def _resident(g, seed, device):
 
 
 @pytest.mark.parametrize("blocks", [(48, 40, 48, 32, 48), (40,)])
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_graph_replays_match_eager(device, precision, blocks):
+@pytest.mark.parametrize("precision,refine", [("fp64", 0), ("mixed", 1), ("mixed", -1)])
+def test_graph_replays_match_eager(device, precision, refine, blocks):
     import torch
 
     g = Geometry(blocks)
@@ -45,7 +45,7 @@ def test_graph_replays_match_eager(device, precision, blocks):
            S.packed_empty(g, E, device, ("d", "u"))]
     ctx = S.Context(device)
     graph = S.SolveGraph(g, A, SL, SG, g_retarded=outs[0], g_lesser=outs[1], g_greater=outs[2],
-                         ctx=ctx, precision=precision, refine=1)
+                         ctx=ctx, precision=precision, refine=refine)
     rng = np.random.default_rng(3)
     for rep in range(3):
         en = rng.uniform(-1, 1, E)
@@ -55,7 +55,7 @@ def test_graph_replays_match_eager(device, precision, blocks):
         ctx.status()
         # eager reference on the same inputs (other buffers), with a sync in between
         kw = {} if precision == "fp64" else dict(precision="mixed",
-                                                 mixed=S.MixedOptions(refine=1, fallback_tol=0.0))
+                                                 mixed=S.MixedOptions(refine=refine, fallback_tol=0.0))
         S.selected_solve(g, A, SL, SG, g_retarded=ref[0], g_lesser=ref[1], g_greater=ref[2], ctx=ctx, **kw)
         ctx.status()
         for X, R in zip(outs, ref):

@@ -148,6 +148,23 @@ def test_mixed_edge_geometries(device, bs):
                 assert rs[e] <= 1e-8 and max(errs) < 1e-3, (e, rs[e], errs)
 
 
+@pytest.mark.parametrize("eta", [1e-2, 1e-3])
+def test_mixed_fp64_schur_mode(device, eta):
+    """refine = -1 computes each inversion's input (the Schur complement and
+    its factor A_ji g_i) in FP64 instead of taking Newton steps: G^R at least
+    as close to the dense FP64 oracle as with one Newton step (the steps'
+    fixed point), G^< and G^> at FP32 level, and still reduced precision."""
+    g, A, SL, SG, t = _case(device, 6, 64, 3, eta, 3005)
+    e1 = _errors(g, _run(device, g, A, SL, SG, t, "mixed", S.MixedOptions(refine=1, fallback_tol=0.0)),
+                 A, SL, SG)
+    es = _errors(g, _run(device, g, A, SL, SG, t, "mixed", S.MixedOptions(refine=-1, fallback_tol=0.0)),
+                 A, SL, SG)
+    assert max(es) < 1e-2 and max(es) > 1e-12, es
+    assert es[0] <= 2.0 * e1[0] + 1e-13, (es, e1)
+    with pytest.raises(ValueError):
+        _run(device, g, A, SL, SG, t, "mixed", S.MixedOptions(refine=-2, fallback_tol=0.0))
+
+
 def test_mixed_residual_matches_cpu_and_flags_fallback(device):
     import torch
 
