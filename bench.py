@@ -266,8 +266,10 @@ def eta_leg(args, model_seed, dev) -> dict:
     In chunks that fit HBM: the FP64 solve (timed), the complex64-product
     solve with one FP64 refinement step and no check (timed), its error
     against the FP64 result per output (max over energies and blocks of the
-    relative Frobenius error per block, per eta), and the mixed solve with the
-    FP64 check, whose fallback fraction at 1e-8 is reported per eta.
+    relative Frobenius error per block, per eta), the same with FP64 Schur
+    complements instead of the refinement step (refine=-1, timed, errors per
+    eta), and the mixed solve with the FP64 check, whose fallback fraction at
+    1e-8 is reported per eta.
     """
     import torch
 
@@ -283,9 +285,10 @@ def eta_leg(args, model_seed, dev) -> dict:
     eta_of = np.array([etas[k % 3] for k in range(ne)])
     ctx = S.default_context(dev)
     errs = {eta: {"G^R": 0.0, "G^<": 0.0, "G^>": 0.0} for eta in etas}
+    errs_s = {eta: {"G^R": 0.0, "G^<": 0.0, "G^>": 0.0} for eta in etas}
     fb = {eta: [] for eta in etas}
     res = {eta: [] for eta in etas}
-    t64 = t32 = 0.0
+    t64 = t32 = ts = 0.0
     for c0 in range(0, ne, args.eta_chunk):
         sl = slice(c0, min(ne, c0 + args.eta_chunk))
         n = sl.stop - sl.start
@@ -308,22 +311,33 @@ def eta_leg(args, model_seed, dev) -> dict:
         t64 += time.perf_counter() - t0
         ctx.status()
         ref = [tuple(p.clone() for p in X if p is not None) for X in outs]
+
+        def accumulate(into):
+            for name, X, R in zip(("G^R", "G^<", "G^>"), outs, ref):
+                worst = torch.zeros(n, dtype=torch.float64, device=dev)
+                for part, rp in zip([p for p in X if p is not None], R):
+                    d = (part - rp).reshape(n, -1, args.block_size ** 2)
+                    r = rp.reshape(n, -1, args.block_size ** 2)
+                    worst = torch.maximum(worst, (torch.linalg.norm(d, dim=-1)
+                                                  / torch.linalg.norm(r, dim=-1)).amax(-1))
+                w = worst.cpu().numpy()
+                for eta in etas:
+                    m = eta_of[sl] == eta
+                    if m.any():
+                        into[eta][name] = max(into[eta][name], float(w[m].max()))
+
         t0 = time.perf_counter()
         solve(precision="mixed", mixed=S.MixedOptions(refine=1, fallback_tol=0.0))
         torch.cuda.synchronize()
         t32 += time.perf_counter() - t0
         ctx.status()
-        for name, X, R in zip(("G^R", "G^<", "G^>"), outs, ref):
-            worst = torch.zeros(n, dtype=torch.float64, device=dev)
-            for part, rp in zip([p for p in X if p is not None], R):
-                d = (part - rp).reshape(n, -1, args.block_size ** 2)
-                r = rp.reshape(n, -1, args.block_size ** 2)
-                worst = torch.maximum(worst, (torch.linalg.norm(d, dim=-1) / torch.linalg.norm(r, dim=-1)).amax(-1))
-            w = worst.cpu().numpy()
-            for eta in etas:
-                m = eta_of[sl] == eta
-                if m.any():
-                    errs[eta][name] = max(errs[eta][name], float(w[m].max()))
+        accumulate(errs)
+        t0 = time.perf_counter()
+        solve(precision="mixed", mixed=S.MixedOptions(refine=-1, fallback_tol=0.0))
+        torch.cuda.synchronize()
+        ts += time.perf_counter() - t0
+        ctx.status()
+        accumulate(errs_s)
         flags = torch.zeros(n, dtype=torch.int32, device=dev)
         resid = torch.zeros(n, dtype=torch.float64, device=dev)
         solve(precision="mixed", mixed=S.MixedOptions(refine=1, fallback_tol=1e-8, fallback_flags=flags,
@@ -341,7 +355,11 @@ def eta_leg(args, model_seed, dev) -> dict:
                        f"{list(etas)}, chunks of {args.eta_chunk}",
            "fp64": {"energies_per_s": round(ne / t64, 2),
                     "frac_fp64_peak": round(f_e * ne / t64 / 1e12 / FP64_PEAK_TFLOPS, 4)},
-           "mixed_unchecked": {"energies_per_s": round(ne / t32, 2), "speedup_vs_fp64": round(t64 / t32, 3)}}
+           "mixed_unchecked": {"energies_per_s": round(ne / t32, 2), "speedup_vs_fp64": round(t64 / t32, 3)},
+           "mixed_fp64_schur": {"energies_per_s": round(ne / ts, 2), "speedup_vs_fp64": round(t64 / ts, 3),
+                                "max_rel_err_per_block": {f"eta={eta:g}": {k: float(f"{v:.3e}")
+                                                                           for k, v in errs_s[eta].items()}
+                                                          for eta in etas}}}
     for eta in etas:
         out[f"eta={eta:g}"] = {"energies": len(fb[eta]),
                                "max_rel_err_per_block": {k: float(f"{v:.3e}") for k, v in errs[eta].items()},
