@@ -135,7 +135,7 @@ class SanchoRubio(OBCSolver):
 
 
 class _IdentityCompressor:
-    """Copies the cached solution (``boundary_system.py:9-40``)."""
+    """Keeps a private copy of the cached solution (no compression)."""
 
     def compress(self, x):
         return x.clone() if hasattr(x, "clone") else (x.copy() if hasattr(x, "copy") else x)
@@ -144,39 +144,61 @@ class _IdentityCompressor:
         return x
 
 
-class OBCSystem:
-    """Memoising OBC boundary system (``obc.py:51-202``,
-    ``boundary_system.py:42-373``) for one process.
+_MEMO_MODES = ("off", "auto", "force", "force-after-first")
 
-    ``system_reducer`` must be ``None`` (the identity reducer): the reference's
-    reducers act on its own array types.
+
+class OBCSystem:
+    """Memoising OBC boundary system for one process, with the interface of
+    the reference's ``OBCSystem`` (``obc.py:51-202`` on
+    ``boundary_system.py:42-373``).
+
+    A call returns ``(solution, rel_residuals, abs_residuals)``.  Where the
+    solution comes from depends on ``memoization_mode``:
+
+    * ``"off"``: the boundary solver, nothing cached;
+    * ``"force-after-first"`` / ``"auto"``: the boundary solver on the first
+      call for a contact, afterwards the cached solution of the last call
+      refined by fixed-point steps;
+    * ``"force"``: like ``"force-after-first"``, but the first call starts the
+      fixed-point steps from ``inv(a_ii)`` instead of calling the solver;
+    * ``"auto"`` re-solves from scratch when the refined cached solution
+      passes the residual test (absolute or relative) on more than
+      ``agreement_threshold`` times the stack-communicator size entries.  That
+      is the reference's test as written (``boundary_system.py:300-313``):
+      a count of converged entries, not a fraction, so with one process a
+      cached solution survives only while no stack entry passes the test.
+
+    Every returned solution has had one more fixed-point step (the one that
+    yields its residuals) and, unless the mode is ``"off"``, is what the next
+    call for the same contact starts from.  ``system_reducer`` must be
+    ``None``: the reference's reducers act on its own array types.
     """
 
     def __init__(self, boundary_solver: OBCSolver, cache_compressor=None, system_reducer=None,
                  num_ref_iterations: int = 2, relative_tol: float = 2e-1,
                  absolute_tol: float = 1e-6, warning_threshold: float = 1e-1,
                  memoization_mode: str = "auto", agreement_threshold: float = 0.999) -> None:
+        compressor = cache_compressor if cache_compressor is not None else _IdentityCompressor()
+        for method in ("compress", "decompress"):
+            if not callable(getattr(compressor, method, None)):
+                raise ValueError(f"cache_compressor needs a {method}() method")
+        if not 0.0 <= agreement_threshold <= 1.0:
+            raise ValueError(f"agreement_threshold must lie in [0, 1], got {agreement_threshold}")
+        if system_reducer is not None:
+            raise NotImplementedError("only the identity system reducer is supported")
+        if num_ref_iterations < 2:
+            warnings.warn(f"num_ref_iterations={num_ref_iterations} raised to 2: the residual "
+                          "test needs at least 2 fixed-point steps", RuntimeWarning)
+            num_ref_iterations = 2
         self.boundary_solver = boundary_solver
+        self.cache_compressor = compressor
         self.num_ref_iterations = num_ref_iterations
         self.relative_tol = relative_tol
         self.absolute_tol = absolute_tol
         self.warning_threshold = warning_threshold
         self.memoization_mode = memoization_mode
         self.agreement_threshold = agreement_threshold
-        self.cache_compressor = cache_compressor or _IdentityCompressor()
-        if not hasattr(self.cache_compressor, "compress"):
-            raise ValueError("Cache compressor must have a 'compress' method.")
-        if not hasattr(self.cache_compressor, "decompress"):
-            raise ValueError("Cache compressor must have a 'decompress' method.")
-        if not (0.0 <= self.agreement_threshold <= 1.0):
-            raise ValueError("Memoizing tolerance must be between 0 and 1.")
-        if system_reducer is not None:
-            raise NotImplementedError("only the identity system reducer is supported")
         self._cache = {}
-        if self.num_ref_iterations < 2:
-            warnings.warn("The number of refinement iterations should be at least 2. Defaulting to 2.",
-                          RuntimeWarning)
-            self.num_ref_iterations = 2
 
     # -- device steps -----------------------------------------------------
     def _refine(self, system, solution, residuals: bool):
@@ -208,39 +230,50 @@ class OBCSystem:
     def _get_residuals(self, boundary_system, test_solution):
         """``(rel, abs, x_ref)`` after one fixed-point step (``obc.py:154-202``)."""
         if not isinstance(test_solution, _torch().Tensor):
-            raise TypeError(f"Expected test_solution to be of type torch.Tensor, got "
-                            f"{type(test_solution)} instead.")
+            raise TypeError(f"test_solution must be a torch.Tensor, not {type(test_solution).__name__}")
         x_ref, abs_res, rel_res = self._refine(boundary_system, test_solution, True)
         return rel_res, abs_res, x_ref
 
-    # -- memoisation (boundary_system.py:196-316) --------------------------
-    def _solve(self, boundary_system, contact: str, **kwargs):
-        solution = self.boundary_solver(*boundary_system, contact, **kwargs)
+    # -- where a solution comes from ----------------------------------------
+    def _from_solver(self, system, contact: str, kwargs):
+        x = self.boundary_solver(*system, contact, **kwargs)
+        self._remember(contact, x)
+        return x
+
+    def _remember(self, contact: str, x) -> None:
         if self.memoization_mode != "off":
-            self._cache[contact] = self.cache_compressor.compress(solution)
-        return solution
+            self._cache[contact] = self.cache_compressor.compress(x)
+
+    def _starting_point(self, system, contact: str):
+        """A solution to refine instead of solving, or None to solve afresh."""
+        mode = self.memoization_mode
+        if mode == "off":
+            return None
+        cached = self.cache_compressor.decompress(self._cache.get(contact))
+        if cached is not None:
+            return cached
+        if mode == "force":
+            return self._get_starting_guess(system)
+        if mode in ("auto", "force-after-first"):
+            return None
+        raise RuntimeError(f"memoization_mode {mode!r} is none of {', '.join(_MEMO_MODES)}")
+
+    def _stack_converged_count(self, rel, ab) -> int:
+        ok = (ab < self.absolute_tol) | (rel < self.relative_tol)
+        return int(ok.sum().item())
 
     def _memoized_solve(self, boundary_system, contact: str, **kwargs):
-        if self.memoization_mode == "off":
-            return self._solve(boundary_system, contact, **kwargs)
-        solution = self.cache_compressor.decompress(self._cache.get(contact, None))
-        if solution is None:
-            if self.memoization_mode in ["auto", "force-after-first"]:
-                return self._solve(boundary_system, contact, **kwargs)
-            elif self.memoization_mode == "force":
-                solution = self._get_starting_guess(boundary_system)
-            else:
-                raise RuntimeError(f"Invalid memoizing mode: {self.memoization_mode}")
-        rel_residuals, abs_residuals, solution = self._get_residuals(boundary_system, solution)
+        start = self._starting_point(boundary_system, contact)
+        if start is None:
+            return self._from_solver(boundary_system, contact, kwargs)
+        rel, ab, x = self._get_residuals(boundary_system, start)
         if self.memoization_mode == "auto":
-            converged = int(((abs_residuals < self.absolute_tol) |
-                             (rel_residuals < self.relative_tol)).sum().item())
-            stack_size = 1  # one process holds the whole stack
-            if converged / stack_size > self.agreement_threshold:
-                return self._solve(boundary_system, contact, **kwargs)
-        for __ in range(self.num_ref_iterations - 2):
-            solution = self._fixed_point_step(boundary_system, solution)
-        return solution
+            ranks = 1  # this process holds the whole stack
+            if self._stack_converged_count(rel, ab) / ranks > self.agreement_threshold:
+                return self._from_solver(boundary_system, contact, kwargs)
+        for _step in range(self.num_ref_iterations - 2):
+            x = self._fixed_point_step(boundary_system, x)
+        return x
 
     def __call__(self, boundary_system, contact: str, **kwargs):
         """Returns ``(solution, rel_residuals, abs_residuals)``
@@ -248,26 +281,20 @@ class OBCSystem:
         staged = [_stage(x) for x in boundary_system]
         lead, from_numpy = staged[0][1], staged[0][2]
         system = tuple(s[0] for s in staged)
-        solution = self._memoized_solve(system, contact, **kwargs)
-        if not isinstance(solution, _torch().Tensor):
-            solution = _stage(solution, system[0].device)[0]
-        solution = solution.reshape(system[0].shape)
-        rel_residuals, abs_residuals, solution = self._get_residuals(system, solution)
-        bad = (rel_residuals > self.warning_threshold) & (abs_residuals > self.absolute_tol)
-        if bool(bad.any().item()):
+        x = self._memoized_solve(system, contact, **kwargs)
+        if not isinstance(x, _torch().Tensor):
+            x = _stage(x, system[0].device)[0]
+        rel, ab, x = self._get_residuals(system, x.reshape(system[0].shape))
+        if bool(((rel > self.warning_threshold) & (ab > self.absolute_tol)).any().item()):
             warnings.warn(
-                f"High error for {contact} of {type(self.boundary_solver).__name__}:\n"
-                f"  Relative recursion error: {rel_residuals.max().item():.3e}\n"
-                f"  Absolute recursion error: {abs_residuals.max().item():.3e}\n",
+                f"{type(self.boundary_solver).__name__} on contact {contact}: fixed-point "
+                f"residual up to {rel.max().item():.3e} relative, {ab.max().item():.3e} absolute",
                 RuntimeWarning)
-        if self.memoization_mode != "off":
-            self._cache[contact] = self.cache_compressor.compress(solution)
-        out = _unstage(solution, lead, from_numpy)
-        rel = rel_residuals.reshape(lead)
-        ab = abs_residuals.reshape(lead)
+        self._remember(contact, x)
+        rel, ab = rel.reshape(lead), ab.reshape(lead)
         if from_numpy:
             rel, ab = rel.cpu().numpy(), ab.cpu().numpy()
-        return out, rel, ab
+        return _unstage(x, lead, from_numpy), rel, ab
 
 
 def compute_contact_obc(m_10, m_00, m_01, occupancies, eta_obc: float, obc, contact: str = "left",

@@ -30,6 +30,8 @@
 // pi(P1) and pi(P2) are zero in S[:, R] before step 5, and rows pi(P2) of T1
 // are zero there, so their P1 contribution enters only through WB.
 // zinv_finish_kernel then writes Out[i][j] = S[pi(i)][pi^{-1}(j)].
+#include <algorithm>
+
 #include "qx_common.h"
 
 namespace qx {
@@ -545,8 +547,13 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
 
 }  // namespace
 
+// Non-decreasing in n: a workspace sized for the largest block of a geometry
+// also holds the need of every smaller block.  Blocks up to 512 use 32-column
+// panels (about 1032 n bytes), larger ones 16-column panels (about 520 n), so
+// for n > 512 the need of a 512-row block can exceed that of n itself.
 size_t zinv_aux_bytes(int n) {
-  return panel_nb(n) == 32 ? PanelAux<32>::bytes(n) : PanelAux<16>::bytes(n);
+  if (panel_nb(n) == 32) return PanelAux<32>::bytes(n);
+  return std::max(PanelAux<32>::bytes(512), PanelAux<16>::bytes(n));
 }
 
 hipError_t launch_zinv(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t sOut, int ldo,

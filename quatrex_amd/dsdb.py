@@ -304,8 +304,17 @@ def _part_sizes(geo: Geometry) -> tuple:
     return (geo.diag_size, geo.offdiag_size, geo.offdiag_size)
 
 
+def structure_of(m):
+    """The object that holds the sparsity structure of ``m``: ``m`` itself, or
+    the parent stack of a substack view (the reference's ``_DStackView`` and
+    :class:`StackView` keep it in ``_dsdbsparse``)."""
+    return getattr(m, "_dsdbsparse", m)
+
+
 def device_map(m, geo: Geometry, device) -> SB.DeviceMap:
-    """The tridiagonal gather map of ``m`` on ``device`` (cached on the object)."""
+    """The tridiagonal gather map of ``m`` on ``device`` (cached on the object
+    that holds the structure)."""
+    m = structure_of(m)
     if hasattr(m, "tridiagonal_map"):
         return m.tridiagonal_map(device)
     cache = m.__dict__.setdefault("_qx_tridiagonal_maps", {})
@@ -379,18 +388,99 @@ def sparsify_batch(m, sl, geo: Geometry, bufs, parts: str, device, ctx) -> None:
 
 
 
+def _compose_first(index, sub, n: int):
+    """Index on the first stack axis equal to applying ``index`` then ``sub``
+    (both a slice with positive step or an int)."""
+    r = range(n)[index]
+    if isinstance(r, int):
+        raise IndexError("a stack view that selects one entry has no stack axis to index")
+    r = r[sub]
+    if isinstance(r, int):
+        return r
+    return slice(r.start, r.start + len(r) * r.step, r.step)
+
+
+class StackView:
+    """A substack of a block-sparse stack, shaped like the reference's
+    ``_DStackView`` (``dsdbsparse.py:1118-1205``): it holds the parent stack
+    (``_dsdbsparse``) and an index on the first stack axis (``_stack_index``),
+    exposes ``data`` as the parent's data at that index (writes reach the
+    parent) and, like the reference's view, carries no ``rows`` / ``cols`` /
+    ``rowptr_map`` of its own."""
+
+    def __init__(self, parent: _DSDB, index):
+        self._dsdbsparse = parent
+        self._stack_index = (index,)
+        self.symmetry = parent.symmetry
+
+    @property
+    def data(self):
+        return self._dsdbsparse.data[self._stack_index]
+
+    @data.setter
+    def data(self, value):
+        self._dsdbsparse.data[self._stack_index] = value
+
+    @property
+    def dtype(self):
+        return self._dsdbsparse.dtype
+
+    @property
+    def block_sizes(self):
+        return self._dsdbsparse.block_sizes
+
+    @property
+    def num_blocks(self) -> int:
+        return self._dsdbsparse.num_blocks
+
+    @property
+    def nnz(self) -> int:
+        return self._dsdbsparse.nnz
+
+    @property
+    def shape(self) -> tuple:
+        n = int(self._dsdbsparse.block_offsets[-1])
+        return (*self.data.shape[:-1], n, n)
+
+    @property
+    def stack(self) -> "_Stack":
+        return _Stack(self)
+
+    @property
+    def blocks(self) -> "_Blocks":
+        return _Blocks(self)
+
+    def get_block(self, data, i: int, j: int):
+        return self._dsdbsparse.get_block(data, i, j)
+
+    def set_block(self, data, i: int, j: int, value) -> None:
+        self._dsdbsparse.set_block(data, i, j, value)
+
+    def to_dense(self):
+        return self._dsdbsparse._structure_view(self.data).to_dense()
+
+
 class _Stack:
-    def __init__(self, m: _DSDB):
+    """``.stack[index]``: a :class:`StackView` over the first stack axis
+    (views of views compose into one view of the parent)."""
+
+    def __init__(self, m):
         self.m = m
 
     def __getitem__(self, sl):
-        if not isinstance(sl, tuple):
-            sl = (sl,)
-        return self.m._structure_view(self.m.data[sl + (Ellipsis,)] if Ellipsis not in sl else self.m.data[sl])
+        if isinstance(sl, tuple):
+            if len(sl) != 1:
+                raise IndexError("stack views index the first stack axis only")
+            sl = sl[0]
+        m = self.m
+        if isinstance(m, StackView):
+            n = int(m._dsdbsparse.data.shape[0])
+            return StackView(m._dsdbsparse, _compose_first(m._stack_index[0], sl, n))
+        return StackView(m, sl)
 
 
 class _Blocks:
-    def __init__(self, m: _DSDB):
+    def __init__(self, m):
         self.m = m
 
     def _ij(self, ij):

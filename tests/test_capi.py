@@ -59,3 +59,27 @@ def test_null_context_is_rejected():
     lib = _lib.load()
     assert lib.rgf_selected_solve_z(None, None, 0, None, None, None, None, None, None, None,
                                     None) == _lib.RGF_EINVAL
+
+
+def _inverse_aux_need(n: int) -> int:
+    """Bytes the blocked inverse of one n x n block uses per energy: two
+    pivot-row buffers of n x NB complex128 values plus two int tables of n
+    (NB = 32 up to 512 rows, 16 above), rounded up to 256 bytes."""
+    nb = 32 if n <= 512 else 16
+    raw = 2 * n * nb * 16 + 2 * n * 4
+    return (raw + 255) // 256 * 256
+
+
+@pytest.mark.parametrize("sizes", [(520, 512), (600, 400, 600), (1024, 300), (700, 513, 512, 700)])
+def test_workspace_holds_every_block_inverse(sizes):
+    """Mixed geometries whose largest block uses 16-column panels and a smaller
+    block 32-column panels: the per-energy inverse scratch must cover the
+    larger need (a 512-row block needs more than a 600-row one)."""
+    from quatrex_amd.layout import Geometry
+    from quatrex_amd.solve import workspace_size
+
+    g = Geometry(sizes)
+    bmax = max(sizes)
+    for E in (1, 3):
+        need = E * (11 * bmax * bmax * 16 + max(_inverse_aux_need(b) for b in sizes))
+        assert workspace_size(g, E) >= need, (sizes, E)

@@ -217,6 +217,24 @@ def test_generator_workload_vs_dense(device, nb, b, ne):
     assert max(errs) <= TOL, errs
 
 
+@pytest.mark.parametrize("sizes", [(520, 512), (600, 400, 600)])
+def test_mixed_panel_widths_vs_dense(device, sizes):
+    """Blocks above 512 rows (16-column inversion panels) next to blocks of
+    at most 512 (32-column panels, whose per-energy scratch is larger): the
+    selected solve and the selected inversion against the dense oracle."""
+    g = Geometry(sizes)
+    _, A, SL, SG = _generator_case(g, 2, W.DEVICE_SEED_BASE + 31)
+    errs = _check_solve(device, g, A, SL, SG)
+    assert max(errs) <= TOL, errs
+    X = S.packed_empty(g, 2, device)
+    S.selected_inv(g, tuple(_t(x, device) for x in A), x=X)
+    S.default_context(device).status()
+    ref = np.linalg.inv(g.to_dense(*A))
+    xd = g.to_dense(*(x.cpu().numpy() for x in X))
+    err = block_rel_errors(xd, ref * tridiagonal_mask(list(sizes)), list(sizes))
+    assert err <= TOL, err
+
+
 def test_mixed_blocks_offdiag_sigma_obc_current(device):
     rng = np.random.default_rng(21)
     bs = [24, 40, 16, 33, 24]
