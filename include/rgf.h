@@ -55,6 +55,10 @@ extern "C" {
                                       lesser/greater products of its forward step              */
 #define RGF_OPT_PROFILE 4          /* 1: time every sweep GEMM launch and every block inversion
                                       with HIP events on the context stream (0: off)          */
+#define RGF_OPT_PERSISTENT 7       /* 1 (default): FP64 selected solves of geometries whose blocks
+                                      are all <= 32 run as one persistent kernel, one workgroup
+                                      per energy with every intermediate block in LDS; 0: the
+                                      launch-per-product-group sweep for every geometry        */
 
 typedef struct rgf_ctx rgf_ctx;
 
@@ -167,6 +171,26 @@ int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energ
                          const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
                          const rgf_bt_const* sigma_greater, const rgf_obc* obc,
                          rgf_bt* g_retarded, rgf_bt* g_lesser, rgf_bt* g_greater, void* current);
+
+/* Several selected solves in one call, e.g. the energy points of different
+ * devices (geometries) that arrived together.  Each problem has the meaning
+ * of one rgf_selected_solve_z call (g_retarded.diag NULL: no retarded output;
+ * upper / lower NULL: not written).  Problems whose blocks are all <= 32 share
+ * ONE persistent kernel launch: one workgroup per energy point with every
+ * intermediate block in LDS, energy points of all those problems pulled
+ * from one device work queue, the most expensive geometries first.  The other
+ * problems run one after the other by the batched sweep.  No reference
+ * counterpart (uneven batching of the serving path; the reference solves one
+ * device per call, rgf.py:115-438). */
+typedef struct rgf_solve_problem {
+  rgf_geometry geom;
+  int64_t n_energies;
+  rgf_bt_const a, sigma_lesser, sigma_greater;
+  const rgf_obc* obc;
+  rgf_bt g_retarded, g_lesser, g_greater;
+  void* current;
+} rgf_solve_problem;
+int rgf_selected_solve_batch_z(rgf_ctx* ctx, int32_t n_problems, const rgf_solve_problem* problems);
 
 /* Mixed-precision selected solve: same interface and outputs (complex128) as
  * rgf_selected_solve_z; the Schur-complement GEMMs and G Sigma G^H triple
@@ -297,20 +321,24 @@ int rgf_boundary_system_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, double et
  * energies (precision 0: FP64; 1: FP32-compute products with FP64
  * inversions and `refine` Newton steps, or FP64 Schur complements with refine
  * = -1, without the a-posteriori check) into
- * a graph that owns its geometry tables and its workspace - or, for FP64,
- * uses the context's caller-provided workspace (rgf_set_workspace) when that
- * is large enough, which must then outlive the graph; rgf_graph_launch
+ * a graph that owns its geometry and job tables and its workspace - or, for
+ * FP64, uses the context's caller-provided workspace (rgf_set_workspace) when
+ * that is large enough, which must then outlive the graph; rgf_graph_launch
  * replays it on the context's stream with whatever data the buffers hold at
- * that time.  The current is not supported in graphs.  The context must stay
- * alive while the graph is launched; rgf_graph_destroy waits for the last
- * launch and may come after rgf_destroy of the context.  No reference
- * counterpart (launch-overhead removal for the serving loop). */
+ * that time.  current: NULL, or device complex128[n_energies][n_blocks+1]
+ * written by every replay (the Meir-Wingreen currents, as in
+ * rgf_selected_solve_z; obc->lesser / obc->greater must then hold blocks 0 and
+ * n_blocks-1).  The context must stay alive while the graph is launched;
+ * rgf_graph_destroy waits for the last launch and may come after rgf_destroy
+ * of the context.  No reference counterpart (launch-overhead removal for the
+ * serving loop). */
 typedef struct rgf_graph rgf_graph;
 int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
                             const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
                             const rgf_bt_const* sigma_greater, const rgf_obc* obc,
                             int32_t precision, int32_t refine, rgf_bt* g_retarded,
-                            rgf_bt* g_lesser, rgf_bt* g_greater, rgf_graph** out);
+                            rgf_bt* g_lesser, rgf_bt* g_greater, void* current,
+                            rgf_graph** out);
 int rgf_graph_launch(rgf_graph* graph);
 int rgf_graph_destroy(rgf_graph* graph);
 

@@ -26,6 +26,7 @@ RGF_OPT_TILE_CFG = 3
 RGF_OPT_PROFILE = 4
 RGF_OPT_UNFUSED = 5
 RGF_OPT_OVERLAP = 6
+RGF_OPT_PERSISTENT = 7
 RGF_OPT_TILE_FOR_BLOCK = 1000  # + block size (FP64 products)
 RGF_OPT_TILE_FOR_BLOCK_F32 = 3000  # + block size (FP32-compute products)
 RGF_NUM_TILE_CFGS = 4
@@ -45,6 +46,7 @@ EXPORTED_SYMBOLS = (
     "rgf_assemble_z",
     "rgf_selected_inv_z",
     "rgf_selected_solve_z",
+    "rgf_selected_solve_batch_z",
     "rgf_selected_solve_mixed",
     "rgf_profile_read",
     "rgf_autotune",
@@ -87,6 +89,21 @@ class MixedOpts_t(ctypes.Structure):
         ("fallback_tol", c_double),
         ("fallback_flags", c_void_p),
         ("residuals", c_void_p),
+    ]
+
+
+class SolveProblem_t(ctypes.Structure):
+    _fields_ = [
+        ("geom", Geometry_t),
+        ("n_energies", c_int64),
+        ("a", BtConst_t),
+        ("sigma_lesser", BtConst_t),
+        ("sigma_greater", BtConst_t),
+        ("obc", POINTER(Obc_t)),
+        ("g_retarded", Bt_t),
+        ("g_lesser", Bt_t),
+        ("g_greater", Bt_t),
+        ("current", c_void_p),
     ]
 
 
@@ -139,6 +156,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.rgf_selected_solve_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
                                          P(BtConst_t), P(BtConst_t), P(Obc_t), P(Bt_t), P(Bt_t),
                                          P(Bt_t), c_void_p]
+    lib.rgf_selected_solve_batch_z.argtypes = [c_void_p, c_int32, P(SolveProblem_t)]
     lib.rgf_selected_solve_mixed.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
                                              P(BtConst_t), P(BtConst_t), P(Obc_t),
                                              P(MixedOpts_t), P(Bt_t), P(Bt_t), P(Bt_t), c_void_p]
@@ -146,7 +164,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.rgf_autotune.argtypes = [c_void_p, c_int64, c_int64, c_int32, P(c_int32), P(ctypes.c_float)]
     lib.rgf_graph_capture_solve.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
                                             P(BtConst_t), P(BtConst_t), P(Obc_t), c_int32, c_int32,
-                                            P(Bt_t), P(Bt_t), P(Bt_t), P(c_void_p)]
+                                            P(Bt_t), P(Bt_t), P(Bt_t), c_void_p, P(c_void_p)]
     lib.rgf_graph_launch.argtypes = [c_void_p]
     lib.rgf_graph_destroy.argtypes = [c_void_p]
     lib.rgf_densify_z.argtypes = [c_void_p, P(Geometry_t), c_int64, c_void_p, c_int64, P(BlockMap_t),

@@ -1,4 +1,6 @@
 // Elementwise and reduction kernels around the RGF sweeps.
+#include <cstring>
+
 #include "qx_common.h"
 
 namespace qx {
@@ -98,6 +100,11 @@ norm_kernel(const NormJob* jobs, double* out, int64_t sOut) {
   if (threadIdx.x == 0) out[(int64_t)e * sOut + blockIdx.x] = sqrt(red[0]);
 }
 
+// One workgroup copies a by-value payload (kernel argument) to device memory.
+__global__ void __launch_bounds__(256) store_words_kernel(uint64_t* dst, WordPayload p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = p.w[i];
+}
+
 __global__ void identity_kernel(double2* X, int n) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx < n * n) X[idx] = cmk((idx / n) == (idx % n) ? 1.0 : 0.0, 0.0);
@@ -173,6 +180,24 @@ hipError_t launch_norms(const NormJob* jobs_dev, int njobs, int nE, double* out,
   if (njobs == 0 || nE == 0) return hipSuccess;
   hipLaunchKernelGGL(norm_kernel, dim3(njobs, nE), dim3(256), 0, stream, jobs_dev, out, sOut);
   return hipGetLastError();
+}
+
+hipError_t launch_store_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+  const size_t words = (bytes + 7) / 8;
+  const char* in = static_cast<const char*>(src);
+  uint64_t* out = static_cast<uint64_t*>(dst);
+  for (size_t w0 = 0; w0 < words; w0 += kPayloadWords) {
+    const size_t n = words - w0 < (size_t)kPayloadWords ? words - w0 : (size_t)kPayloadWords;
+    WordPayload p;
+    const size_t off = w0 * 8;
+    const size_t take = bytes - off < n * 8 ? bytes - off : n * 8;
+    std::memcpy(p.w, in + off, take);
+    if (take < n * 8) std::memset(reinterpret_cast<char*>(p.w) + take, 0, n * 8 - take);
+    hipLaunchKernelGGL(store_words_kernel, dim3(1), dim3(256), 0, stream, out + w0, p, (int)n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_identity(double2* X, int n, hipStream_t stream) {

@@ -139,6 +139,17 @@ struct NormJob {
 };
 hipError_t launch_norms(const NormJob* jobs_dev, int njobs, int nE, double* out, int64_t sOut,
                         hipStream_t stream);
+// Host tables (geometry offsets, trace / norm job lists) reach the device as
+// kernel arguments: launch_store_bytes copies `bytes` from host memory to
+// dst (8-byte aligned; the last word is written whole, so reserve bytes
+// rounded up to 8) through by-value payloads of up to 2 KiB per launch.  No
+// host buffer has to outlive the call and no host synchronisation is needed,
+// so the copy also works inside a captured graph.
+constexpr int kPayloadWords = 256;
+struct WordPayload {
+  uint64_t w[kPayloadWords];
+};
+hipError_t launch_store_bytes(void* dst, const void* src, size_t bytes, hipStream_t stream);
 // n x n identity (one matrix; use energy stride 0 to share it).
 hipError_t launch_identity(double2* X, int n, hipStream_t stream);
 
@@ -153,6 +164,37 @@ hipError_t launch_z2c(const double2* src, int64_t sSrc, float2* dst, int64_t sDs
                       int nE, hipStream_t stream);
 hipError_t launch_c2z(const float2* src, int64_t sSrc, double2* dst, int64_t sDst, int64_t count,
                       int nE, hipStream_t stream);
+
+// Persistent on-chip selected solve for geometries whose blocks are all at
+// most kSmallMaxBlock (sweep_small.hip).  One problem = one geometry with
+// its energy batch and packed buffers (energy strides Dsz / Usz; block
+// tables on the device); jobs job0 .. job0 + nE - 1 of the work queue are its
+// energies.  Per-workgroup scratch holds the left-connected g_i when the
+// retarded output is absent and the Sigma~ blocks of the current.
+constexpr int kSmallMaxBlock = 32;
+struct SmallProblem {
+  int32_t N, nE, job0, pad_;
+  int64_t e_base, Dsz, Usz;
+  const int* bs;
+  const int64_t* od;
+  const int64_t* ou;
+  const double2 *Ad, *Au, *Al, *SLd, *SLu, *SGd, *SGu;
+  double2 *GRd, *GRu, *GRl, *GLd, *GLu, *GLl, *GGd, *GGu, *GGl;
+  const double2* const* obr;
+  const double2* const* obl;
+  const double2* const* obg;
+  double2* cur;  // [nE][N+1] or null
+};
+struct SmallLaunch {
+  int32_t nprob, total_jobs;
+  int* counter;  // work-queue head, reset by the launcher
+  int* info;     // singular-pivot status word
+  char* scratch;
+  int64_t scratch_per_wg;
+  const SmallProblem* probs;  // device array of nprob problems
+};
+size_t sweep_small_scratch_per_wg(int n_blocks, bool need_g, bool current);
+hipError_t launch_sweep_small(const SmallLaunch& L, int grid, hipStream_t stream);
 
 // Block-sparse <-> packed tridiagonal (blocks.hip).  Entry k of a map moves
 // data[e][src[k]] to out[e][dst[k]] (densify) or back (sparsify, stored

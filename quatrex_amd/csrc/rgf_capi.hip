@@ -345,6 +345,14 @@ constexpr int kMaxNormJobs = 64;
 
 constexpr int kSlots = 11;
 
+// Table bytes of one problem of the persistent sweep: its descriptor, block
+// sizes and offsets, and the three OBC pointer lists.
+size_t small_table_bytes(const Geo& g) {
+  auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return r(sizeof(SmallProblem)) + r(g.N * sizeof(int)) + 2 * r(g.N * sizeof(int64_t)) +
+         r(3 * g.N * sizeof(void*));
+}
+
 size_t work_bytes(const Geo& g, int64_t E, bool solve, bool current, bool have_gr,
                   bool mixed = false, bool have_gr_off = true) {
   const size_t bb = (size_t)g.bmax * g.bmax * sizeof(double2);
@@ -732,17 +740,17 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
       job(ST(w.stg, j), GL.D(j), GG.D(j), ST(w.stl, j), (int)g.bs[j], j, 1.0);
     job(OG(0), GL.D(0), GG.D(0), OL(0), (int)g.bs[0], 0, 1.0);
     job(OG(N - 1), GL.D(N - 1), GG.D(N - 1), OL(N - 1), (int)g.bs[N - 1], N, -1.0);
-    const size_t bytes = jobs.size() * sizeof(TraceJob);
-    if (c->meta_used + bytes > c->meta_bytes) return c->fail(RGF_EHIP, "trace job area too small");
-    TraceJob* dev = reinterpret_cast<TraceJob*>(static_cast<char*>(c->meta) + c->meta_used);
-    rc = c->hip(hipMemcpyAsync(dev, jobs.data(), bytes, hipMemcpyHostToDevice, c->stream),
-                "copy trace jobs");
+    // the job list is written by a kernel with the list as its argument: no
+    // host wait, and a captured graph rewrites it on every replay; the area
+    // is reused by the next chunk (stream order)
+    const size_t mark = c->tables_used;
+    void* dev = nullptr;
+    rc = c->put_table(jobs.data(), jobs.size() * sizeof(TraceJob), &dev);
     if (rc) return rc;
-    rc = c->hip(hipStreamSynchronize(c->stream), "sync trace jobs");  // host vector lifetime
-    if (rc) return rc;
-    rc = c->hip(launch_traces(dev, (int)jobs.size(), E, io.current + e0 * (N + 1), N + 1,
-                              c->stream),
+    rc = c->hip(launch_traces(static_cast<TraceJob*>(dev), (int)jobs.size(), E,
+                              io.current + e0 * (N + 1), N + 1, c->stream),
                 "trace launch");
+    c->tables_used = mark;
     if (rc) return rc;
   }
 
@@ -754,6 +762,144 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
   rc = c->hip(launch_project(GG.d + e0 * g.Dsz, g.Dsz, c->offs_dev, c->sizes_dev, N, (int)g.bmax, E,
                              c->stream),
               "project launch");
+  return rc;
+}
+
+}  // namespace
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Persistent on-chip sweep (sweep_small.hip) for geometries whose blocks are
+// all at most kSmallMaxBlock
+// ---------------------------------------------------------------------------
+
+// Largest persistent grid assumed when sizing workspaces without a context.
+constexpr int kMaxPersistentGrid = 1024;
+
+// One problem of a persistent launch: a geometry, its energy count and the
+// energy-0 views of its buffers.
+struct SmallSpec {
+  const Geo* g;
+  int64_t nE;
+  SolveIO io;
+};
+
+bool small_geometry(const Geo& g) { return g.bmax <= kSmallMaxBlock; }
+
+size_t small_per_wg(const Geo& g, bool have_gr, bool current) {
+  return sweep_small_scratch_per_wg(g.N, !have_gr, current);
+}
+
+// Workspace of a persistent launch: per-workgroup scratch for at most
+// min(energies, grid) workgroups.
+size_t small_work_bytes(const Geo& g, int64_t nE, bool have_gr, bool current, int grid) {
+  const size_t per = small_per_wg(g, have_gr, current);
+  if (per == 0) return 0;
+  return per * (size_t)std::min<int64_t>(nE, grid) + 256;
+}
+
+// Relative cost of one energy of a geometry (block steps x block volume).
+double small_cost(const Geo& g) {
+  double c = 0;
+  for (int i = 0; i < g.N; ++i) c += (double)g.bs[i] * g.bs[i] * g.bs[i];
+  return c;
+}
+
+// Tables of every problem go to the table area (no host wait), then one
+// launch: the work queue holds the energies of all problems, the most
+// expensive problems first.
+int run_small(Ctx* c, std::vector<SmallSpec> specs, char* scratch) {
+  std::stable_sort(specs.begin(), specs.end(), [](const SmallSpec& a, const SmallSpec& b) {
+    return small_cost(*a.g) > small_cost(*b.g);
+  });
+  size_t need = 256;
+  int64_t total = 0;
+  size_t per_wg = 0;
+  for (const SmallSpec& sp : specs) {
+    need += small_table_bytes(*sp.g);
+    total += sp.nE;
+    per_wg = std::max(per_wg, small_per_wg(*sp.g, sp.io.GR.d != nullptr, sp.io.current != nullptr));
+  }
+  if (total == 0) return RGF_OK;
+  if (total > 0x7fffffff) return c->fail(RGF_EUNSUPPORTED, "persistent sweep: too many energies");
+  if (!c->tables_fixed) {
+    int rc = c->ensure_meta(need);
+    if (rc) return rc;
+    c->tables = static_cast<char*>(c->meta);
+    c->tables_cap = c->meta_bytes;
+    c->tables_used = 0;
+  }
+  const size_t mark = c->tables_used;
+  std::vector<SmallProblem> probs;
+  int rc = RGF_OK;
+  int job0 = 0;
+  for (const SmallSpec& sp : specs) {
+    const Geo& g = *sp.g;
+    const SolveIO& io = sp.io;
+    std::vector<int> bs(g.N);
+    for (int i = 0; i < g.N; ++i) bs[i] = (int)g.bs[i];
+    void *d_bs = nullptr, *d_od = nullptr, *d_ou = nullptr;
+    rc = c->put_table(bs.data(), g.N * sizeof(int), &d_bs);
+    if (!rc) rc = c->put_table(g.od.data(), g.N * sizeof(int64_t), &d_od);
+    if (!rc) rc = c->put_table(g.ou.data(), g.N * sizeof(int64_t), &d_ou);
+    void* d_ob[3] = {nullptr, nullptr, nullptr};
+    if (!rc && io.obc) {
+      const void* const* lists[3] = {io.obc->retarded, io.obc->lesser, io.obc->greater};
+      std::vector<const void*> all(3 * g.N, nullptr);
+      for (int k = 0; k < 3; ++k)
+        for (int i = 0; lists[k] && i < g.N; ++i) all[k * g.N + i] = lists[k][i];
+      void* d_all = nullptr;
+      rc = c->put_table(all.data(), all.size() * sizeof(void*), &d_all);
+      for (int k = 0; !rc && k < 3; ++k)
+        if (lists[k]) d_ob[k] = static_cast<char*>(d_all) + k * g.N * sizeof(void*);
+    }
+    if (rc) {
+      c->tables_used = mark;
+      return rc;
+    }
+    SmallProblem q{};
+    q.N = g.N;
+    q.nE = (int32_t)sp.nE;
+    q.job0 = job0;
+    q.e_base = 0;
+    q.Dsz = g.Dsz;
+    q.Usz = g.Usz;
+    q.bs = static_cast<const int*>(d_bs);
+    q.od = static_cast<const int64_t*>(d_od);
+    q.ou = static_cast<const int64_t*>(d_ou);
+    q.Ad = io.A.d; q.Au = io.A.u; q.Al = io.A.l;
+    q.SLd = io.SL.d; q.SLu = io.SL.u;
+    q.SGd = io.SG.d; q.SGu = io.SG.u;
+    q.GRd = io.GR.d; q.GRu = io.GR.u; q.GRl = io.GR.l;
+    q.GLd = io.GL.d; q.GLu = io.GL.u; q.GLl = io.GL.l;
+    q.GGd = io.GG.d; q.GGu = io.GG.u; q.GGl = io.GG.l;
+    q.obr = static_cast<const double2* const*>(d_ob[0]);
+    q.obl = static_cast<const double2* const*>(d_ob[1]);
+    q.obg = static_cast<const double2* const*>(d_ob[2]);
+    q.cur = io.current;
+    probs.push_back(q);
+    job0 += (int)sp.nE;
+  }
+  void *d_probs = nullptr, *d_counter = nullptr;
+  const int zero = 0;
+  rc = c->put_table(probs.data(), probs.size() * sizeof(SmallProblem), &d_probs);
+  if (!rc) rc = c->put_table(&zero, sizeof(int), &d_counter);
+  if (rc) {
+    c->tables_used = mark;
+    return rc;
+  }
+  SmallLaunch L{};
+  L.nprob = (int)probs.size();
+  L.total_jobs = (int)total;
+  L.counter = static_cast<int*>(d_counter);
+  L.info = c->info_dev;
+  L.scratch = per_wg ? scratch : nullptr;
+  L.scratch_per_wg = (int64_t)per_wg;
+  L.probs = static_cast<const SmallProblem*>(d_probs);
+  const int grid = (int)std::min<int64_t>({total, (int64_t)c->n_cu, (int64_t)kMaxPersistentGrid});
+  rc = c->hip(launch_sweep_small(L, grid, c->stream), "persistent sweep launch");
+  c->tables_used = mark;
   return rc;
 }
 
@@ -847,16 +993,14 @@ int check_chunk(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io, cons
     nj(io.GG.D(i), b, b);
     nj(io.SL.D(i), b, b);
     nj(io.SG.D(i), b, b);
-    const size_t bytes = jobs.size() * sizeof(NormJob);
-    if (c->meta_used + bytes > c->meta_bytes) return c->fail(RGF_EHIP, "norm job area too small");
-    NormJob* dev = reinterpret_cast<NormJob*>(static_cast<char*>(c->meta) + c->meta_used);
-    rc = c->hip(hipMemcpyAsync(dev, jobs.data(), bytes, hipMemcpyHostToDevice, c->stream), "norm jobs");
+    const size_t mark = c->tables_used;
+    void* dev = nullptr;
+    rc = c->put_table(jobs.data(), jobs.size() * sizeof(NormJob), &dev);
     if (rc) return rc;
-    rc = c->hip(hipStreamSynchronize(c->stream), "norm jobs sync");
-    if (rc) return rc;
-    rc = c->hip(launch_norms(dev, (int)jobs.size(), E, w.norms + bi_ * per_block, kMaxNormJobs,
-                             c->stream),
+    rc = c->hip(launch_norms(static_cast<NormJob*>(dev), (int)jobs.size(), E,
+                             w.norms + bi_ * per_block, kMaxNormJobs, c->stream),
                 "norm launch");
+    c->tables_used = mark;
     if (rc) return rc;
   }
   rc = c->hip(hipMemcpyAsync(h.data(), w.norms, h.size() * sizeof(double), hipMemcpyDeviceToHost,
@@ -892,24 +1036,53 @@ int Ctx::reset_info() {
   return hip(hipGetLastError(), "zero info");
 }
 
-// Uploads block offsets/sizes used by the projection kernel (meta area, start).
-int Ctx::upload_geometry(const Geo& g) {
-  const size_t bytes = g.N * (sizeof(int64_t) + sizeof(int));
-  // geometry tables first, then room for one chunk's trace jobs
-  int rc = ensure_meta(bytes + 256 + (g.N + 2) * sizeof(TraceJob) + 4096);
+// Bytes of table memory one call may use: the geometry tables (block
+// offsets and sizes), then room for one chunk's job list (trace jobs, or the
+// norm jobs of the mixed-precision check, or the boundary table of the
+// assembly), each rounded up to 256 bytes.
+size_t table_bytes(const Geo& g) {
+  auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t geo = r(g.N * (sizeof(int64_t) + sizeof(int)));
+  const size_t jobs = std::max({(g.N + 2) * sizeof(TraceJob), 16 * sizeof(NormJob),
+                                g.N * sizeof(double2*), small_table_bytes(g) + 256});
+  return geo + r(jobs);
+}
+
+int Ctx::put_table(const void* host, size_t bytes, void** dev) {
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (!tables || tables_used + need > tables_cap)
+    return fail(RGF_EHIP, "table area too small (" + std::to_string(tables_used + need) + " > " +
+                              std::to_string(tables_cap) + " bytes)");
+  char* d = tables + tables_used;
+  int rc = hip(launch_store_bytes(d, host, bytes, stream), "table upload");
   if (rc) return rc;
+  tables_used += need;
+  *dev = d;
+  return RGF_OK;
+}
+
+// Block offsets/sizes used by the projection and assembly kernels, at the
+// start of the table area.
+int Ctx::upload_geometry(const Geo& g) {
+  const size_t need = table_bytes(g);
+  if (!tables_fixed) {
+    int rc = ensure_meta(need);
+    if (rc) return rc;
+    tables = static_cast<char*>(meta);
+    tables_cap = meta_bytes;
+  }
+  tables_used = 0;
+  const size_t bytes = g.N * (sizeof(int64_t) + sizeof(int));
   std::vector<char> host(bytes);
   std::memcpy(host.data(), g.od.data(), g.N * sizeof(int64_t));
   std::vector<int> sz(g.N);
   for (int i = 0; i < g.N; ++i) sz[i] = (int)g.bs[i];
   std::memcpy(host.data() + g.N * sizeof(int64_t), sz.data(), g.N * sizeof(int));
-  rc = hip(hipMemcpyAsync(meta, host.data(), bytes, hipMemcpyHostToDevice, stream), "geometry upload");
+  void* dev = nullptr;
+  int rc = put_table(host.data(), bytes, &dev);
   if (rc) return rc;
-  rc = hip(hipStreamSynchronize(stream), "geometry upload sync");
-  if (rc) return rc;
-  offs_dev = reinterpret_cast<int64_t*>(meta);
-  sizes_dev = reinterpret_cast<int*>(static_cast<char*>(meta) + g.N * sizeof(int64_t));
-  meta_used = (bytes + 255) & ~(size_t)255;
+  offs_dev = static_cast<int64_t*>(dev);
+  sizes_dev = reinterpret_cast<int*>(static_cast<char*>(dev) + g.N * sizeof(int64_t));
   return RGF_OK;
 }
 
@@ -928,6 +1101,108 @@ int64_t Ctx::chunk_for(const Geo& g, int64_t nE, bool solve, bool current, bool 
 }
 
 }  // namespace qx
+
+namespace {
+
+// The table-area fields of a context, saved while a graph capture or the
+// allocation-free entry point points them elsewhere.
+struct TableState {
+  char* tables;
+  size_t cap, used;
+  bool fixed;
+  int64_t* offs;
+  int* sizes;
+  static TableState save(const qx::Ctx& c) {
+    return TableState{c.tables, c.tables_cap, c.tables_used, c.tables_fixed, c.offs_dev, c.sizes_dev};
+  }
+  void restore(qx::Ctx* c) const {
+    c->tables = tables;
+    c->tables_cap = cap;
+    c->tables_used = used;
+    c->tables_fixed = fixed;
+    c->offs_dev = offs;
+    c->sizes_dev = sizes;
+  }
+};
+
+// The current needs the lesser and greater boundary blocks of both ends
+// (rgf.py:414-428; the reference raises TypeError without them).
+bool current_obc_ok(const rgf_obc* obc, const qx::Geo& g) {
+  const int last = g.N - 1;
+  return obc && obc->lesser && obc->greater && obc->lesser[0] && obc->greater[0] &&
+         obc->lesser[last] && obc->greater[last];
+}
+
+// Argument checks shared by the selected-solve entry points.
+int check_solve_args(qx::Ctx* ctx, const qx::Geo& g, const rgf_bt_const* a,
+                     const rgf_bt_const* sigma_lesser, const rgf_bt_const* sigma_greater,
+                     const rgf_obc* obc, const rgf_bt* g_lesser, const rgf_bt* g_greater,
+                     const void* current) {
+  const bool multi = g.N > 1;
+  if (!a || !a->diag || (multi && (!a->upper || !a->lower)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: A needs diag, upper and lower");
+  if (!sigma_lesser || !sigma_greater || !sigma_lesser->diag || !sigma_greater->diag ||
+      (multi && (!sigma_lesser->upper || !sigma_greater->upper)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: Sigma needs diag and upper");
+  if (!g_lesser || !g_greater || !g_lesser->diag || !g_greater->diag ||
+      (multi && (!g_lesser->upper || !g_greater->upper)))
+    return ctx->fail(RGF_EINVAL, "selected_solve: G^< and G^> need diag and upper outputs");
+  if (current && !current_obc_ok(obc, g))
+    return ctx->fail(RGF_EINVAL,
+                     "current needs lesser/greater OBC blocks on the first and last block");
+  return RGF_OK;
+}
+
+// One chunk's worth of solve I/O views, shared by the FP64 entry points.
+SolveIO solve_io(const qx::Geo& g, int64_t e0, const rgf_bt_const* a,
+                 const rgf_bt_const* sigma_lesser, const rgf_bt_const* sigma_greater,
+                 const rgf_obc* obc, rgf_bt* g_retarded, rgf_bt* g_lesser, rgf_bt* g_greater,
+                 void* current) {
+  const bool have_gr = g_retarded && g_retarded->diag;
+  SolveIO io;
+  io.A = view(g, a->diag, a->upper, a->lower, e0);
+  io.SL = view(g, sigma_lesser->diag, sigma_lesser->upper, nullptr, e0);
+  io.SG = view(g, sigma_greater->diag, sigma_greater->upper, nullptr, e0);
+  io.obc = obc;
+  io.GR = have_gr ? view(g, g_retarded->diag, g_retarded->upper, g_retarded->lower, e0)
+                  : BTV{&g, nullptr, nullptr, nullptr, 0, -1, -1};
+  io.GL = view(g, g_lesser->diag, g_lesser->upper, g_lesser->lower, e0);
+  io.GG = view(g, g_greater->diag, g_greater->upper, g_greater->lower, e0);
+  io.current = static_cast<double2*>(current);
+  return io;
+}
+
+}  // namespace
+
+namespace {
+
+// FP64 selected solve by the launch-per-product-group sweep, all energies in
+// chunks that fit the workspace budget (the status word is not reset here).
+int solve_chunked(qx::Ctx* ctx, const qx::Geo& g, int64_t n_energies, const rgf_bt_const* a,
+                  const rgf_bt_const* sigma_lesser, const rgf_bt_const* sigma_greater,
+                  const rgf_obc* obc, rgf_bt* g_retarded, rgf_bt* g_lesser, rgf_bt* g_greater,
+                  void* current) {
+  const bool have_gr = g_retarded && g_retarded->diag;
+  const int64_t chunk = ctx->chunk_for(g, n_energies, true, current != nullptr, have_gr);
+  int rc = ctx->upload_geometry(g);
+  if (rc) return rc;
+  char* ws = nullptr;
+  rc = ctx->ensure_workspace(work_bytes(g, chunk, true, current != nullptr, have_gr), &ws);
+  if (rc) return rc;
+  Work w;
+  carve(g, chunk, true, current != nullptr, have_gr, ws, &w);
+  for (int64_t e0 = 0; e0 < n_energies; e0 += chunk) {
+    const int E = (int)std::min<int64_t>(chunk, n_energies - e0);
+    const SolveIO io = solve_io(g, e0, a, sigma_lesser, sigma_greater, obc, g_retarded, g_lesser,
+                                g_greater, current);
+    // workspace-resident retarded storage is indexed from the chunk start
+    rc = sweep_chunk_solve(ctx, g, e0, E, io, w);
+    if (rc) return rc;
+  }
+  return RGF_OK;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -988,6 +1263,9 @@ int rgf_create(int device, void* stream, rgf_ctx** out) {
     delete ctx;
     return RGF_EHIP;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    ctx->n_cu = cus;
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
     ctx->ws_budget = std::min<size_t>((size_t)16 << 30, free_b / 4);
@@ -1020,6 +1298,7 @@ int rgf_set_option(rgf_ctx* ctx, int32_t option, int64_t value) {
     case RGF_OPT_PROFILE: ctx->profile = value != 0; return RGF_OK;
     case RGF_OPT_UNFUSED: ctx->unfused = value != 0; return RGF_OK;
     case RGF_OPT_OVERLAP: ctx->overlap = value != 0; return RGF_OK;
+    case RGF_OPT_PERSISTENT: ctx->persistent = value != 0; return RGF_OK;
     case RGF_OPT_TILE_CFG:
       if (value >= qx::kNumTileCfgs) return ctx->fail(RGF_EINVAL, "unknown tile config");
       ctx->tile_override = (int)value;
@@ -1079,6 +1358,9 @@ int rgf_workspace_size(const rgf_geometry* geom, int64_t n_energies, int32_t sol
   int rc = g.init(geom, &err);
   if (rc) return rc;
   *bytes = work_bytes(g, n_energies, solve != 0, with_current != 0, with_retarded != 0);
+  if (solve && small_geometry(g))
+    *bytes = std::max(*bytes, small_work_bytes(g, n_energies, with_retarded != 0, with_current != 0,
+                                               kMaxPersistentGrid));
   return RGF_OK;
 }
 
@@ -1143,17 +1425,14 @@ int rgf_assemble_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies, c
   if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
   rc = ctx->upload_geometry(g);
   if (rc) return rc;
-  // per-block boundary pointers go to the meta area after the geometry tables
+  // per-block boundary pointers go to the table area after the geometry tables
   std::vector<const double2*> sig(g.N, nullptr);
   if (sigma_b)
     for (int i = 0; i < g.N; ++i) sig[i] = static_cast<const double2*>(sigma_b[i]);
-  const size_t bytes = g.N * sizeof(double2*);
-  if (ctx->meta_used + bytes > ctx->meta_bytes) return ctx->fail(RGF_EHIP, "meta area too small");
-  const double2** dsig = reinterpret_cast<const double2**>(static_cast<char*>(ctx->meta) + ctx->meta_used);
-  rc = ctx->hip(hipMemcpyAsync(dsig, sig.data(), bytes, hipMemcpyHostToDevice, ctx->stream), "sigma_b table");
+  void* dev = nullptr;
+  rc = ctx->put_table(sig.data(), g.N * sizeof(double2*), &dev);
   if (rc) return rc;
-  rc = ctx->hip(hipStreamSynchronize(ctx->stream), "sigma_b table sync");
-  if (rc) return rc;
+  const double2** dsig = static_cast<const double2**>(dev);
   return ctx->hip(launch_assemble((const double2*)h->diag, (const double2*)h->upper,
                                   (const double2*)h->lower, g.Dsz, g.Usz, (int)n_energies,
                                   (const double2*)z, ctx->offs_dev, ctx->sizes_dev, dsig, g.N,
@@ -1206,48 +1485,76 @@ int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energ
   if (rc) return rc;
   if (n_energies < 0) return ctx->fail(RGF_EINVAL, "n_energies < 0");
   if (n_energies == 0) return RGF_OK;
-  const bool multi = g.N > 1;
-  if (!a || !a->diag || (multi && (!a->upper || !a->lower)))
-    return ctx->fail(RGF_EINVAL, "selected_solve: A needs diag, upper and lower");
-  if (!sigma_lesser || !sigma_greater || !sigma_lesser->diag || !sigma_greater->diag ||
-      (multi && (!sigma_lesser->upper || !sigma_greater->upper)))
-    return ctx->fail(RGF_EINVAL, "selected_solve: Sigma needs diag and upper");
-  if (!g_lesser || !g_greater || !g_lesser->diag || !g_greater->diag ||
-      (multi && (!g_lesser->upper || !g_greater->upper)))
-    return ctx->fail(RGF_EINVAL, "selected_solve: G^< and G^> need diag and upper outputs");
-  if (current) {
-    const int last = g.N - 1;
-    if (!obc || !obc->lesser || !obc->greater || !obc->lesser[0] || !obc->greater[0] ||
-        !obc->lesser[last] || !obc->greater[last])
-      return ctx->fail(RGF_EINVAL,
-                       "current needs lesser/greater OBC blocks on the first and last block");
-  }
+  rc = check_solve_args(ctx, g, a, sigma_lesser, sigma_greater, obc, g_lesser, g_greater, current);
+  if (rc) return rc;
   if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
   const bool have_gr = g_retarded && g_retarded->diag;
-  const int64_t chunk = ctx->chunk_for(g, n_energies, true, current != nullptr, have_gr);
-  rc = ctx->upload_geometry(g);
-  if (rc) return rc;
-  char* ws = nullptr;
-  rc = ctx->ensure_workspace(work_bytes(g, chunk, true, current != nullptr, have_gr), &ws);
-  if (rc) return rc;
   rc = ctx->reset_info();
   if (rc) return rc;
-  Work w;
-  carve(g, chunk, true, current != nullptr, have_gr, ws, &w);
-  for (int64_t e0 = 0; e0 < n_energies; e0 += chunk) {
-    const int E = (int)std::min<int64_t>(chunk, n_energies - e0);
-    SolveIO io;
-    io.A = view(g, a->diag, a->upper, a->lower, e0);
-    io.SL = view(g, sigma_lesser->diag, sigma_lesser->upper, nullptr, e0);
-    io.SG = view(g, sigma_greater->diag, sigma_greater->upper, nullptr, e0);
-    io.obc = obc;
-    io.GR = have_gr ? view(g, g_retarded->diag, g_retarded->upper, g_retarded->lower, e0)
-                    : BTV{&g, nullptr, nullptr, nullptr, 0, -1, -1};
-    io.GL = view(g, g_lesser->diag, g_lesser->upper, g_lesser->lower, e0);
-    io.GG = view(g, g_greater->diag, g_greater->upper, g_greater->lower, e0);
-    io.current = static_cast<double2*>(current);
-    // workspace-resident retarded storage is indexed from the chunk start
-    rc = sweep_chunk_solve(ctx, g, e0, E, io, w);
+  if (ctx->persistent && small_geometry(g)) {
+    // every block <= 32: one persistent launch, intermediates in LDS
+    const size_t bytes = small_work_bytes(g, n_energies, have_gr, current != nullptr, ctx->n_cu);
+    char* ws = nullptr;
+    if (bytes) {
+      rc = ctx->ensure_workspace(bytes, &ws);
+      if (rc) return rc;
+    }
+    return run_small(ctx, {SmallSpec{&g, n_energies, solve_io(g, 0, a, sigma_lesser, sigma_greater, obc,
+                                                               g_retarded, g_lesser, g_greater, current)}},
+                     ws);
+  }
+  return solve_chunked(ctx, g, n_energies, a, sigma_lesser, sigma_greater, obc, g_retarded, g_lesser,
+                       g_greater, current);
+  GUARD_END
+}
+
+int rgf_selected_solve_batch_z(rgf_ctx* ctx, int32_t n_problems, const rgf_solve_problem* problems) {
+  if (!ctx) return RGF_EINVAL;
+  GUARD_BEGIN
+  if (n_problems < 0 || (n_problems > 0 && !problems)) return ctx->fail(RGF_EINVAL, "batch: problems required");
+  std::vector<qx::Geo> geos(n_problems);
+  for (int p = 0; p < n_problems; ++p) {
+    const rgf_solve_problem& q = problems[p];
+    int rc = geos[p].init(&q.geom, &ctx->last_error);
+    if (rc) return rc;
+    if (q.n_energies < 0) return ctx->fail(RGF_EINVAL, "batch: n_energies < 0");
+    rc = check_solve_args(ctx, geos[p], &q.a, &q.sigma_lesser, &q.sigma_greater, q.obc, &q.g_lesser,
+                          &q.g_greater, q.current);
+    if (rc) return rc;
+  }
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  int rc = ctx->reset_info();
+  if (rc) return rc;
+  // geometries with every block <= 32 share one persistent launch
+  std::vector<SmallSpec> small;
+  size_t per_wg = 0;
+  int64_t small_energies = 0;
+  for (int p = 0; p < n_problems; ++p) {
+    const rgf_solve_problem& q = problems[p];
+    if (!ctx->persistent || !small_geometry(geos[p]) || q.n_energies == 0) continue;
+    rgf_bt gr = q.g_retarded, gl = q.g_lesser, gg = q.g_greater;
+    small.push_back(SmallSpec{&geos[p], q.n_energies,
+                              solve_io(geos[p], 0, &q.a, &q.sigma_lesser, &q.sigma_greater, q.obc,
+                                       gr.diag ? &gr : nullptr, &gl, &gg, q.current)});
+    per_wg = std::max(per_wg, small_per_wg(geos[p], gr.diag != nullptr, q.current != nullptr));
+    small_energies += q.n_energies;
+  }
+  if (!small.empty()) {
+    char* ws = nullptr;
+    if (per_wg) {
+      const size_t grid = (size_t)std::min<int64_t>(small_energies, ctx->n_cu);
+      rc = ctx->ensure_workspace(per_wg * grid + 256, &ws);
+      if (rc) return rc;
+    }
+    rc = run_small(ctx, small, ws);
+    if (rc) return rc;
+  }
+  for (int p = 0; p < n_problems; ++p) {
+    const rgf_solve_problem& q = problems[p];
+    if ((ctx->persistent && small_geometry(geos[p])) || q.n_energies == 0) continue;
+    rgf_bt gr = q.g_retarded, gl = q.g_lesser, gg = q.g_greater;
+    rc = solve_chunked(ctx, geos[p], q.n_energies, &q.a, &q.sigma_lesser, &q.sigma_greater, q.obc,
+                       gr.diag ? &gr : nullptr, &gl, &gg, q.current);
     if (rc) return rc;
   }
   return RGF_OK;
@@ -1335,7 +1642,8 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
                             const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
                             const rgf_bt_const* sigma_greater, const rgf_obc* obc,
                             int32_t precision, int32_t refine, rgf_bt* g_retarded,
-                            rgf_bt* g_lesser, rgf_bt* g_greater, rgf_graph** out) {
+                            rgf_bt* g_lesser, rgf_bt* g_greater, void* current,
+                            rgf_graph** out) {
   if (!ctx || !out) return RGF_EINVAL;
   *out = nullptr;
   GUARD_BEGIN
@@ -1351,6 +1659,8 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
       (multi && (!sigma_lesser->upper || !sigma_greater->upper)) || !g_lesser || !g_greater ||
       !g_lesser->diag || !g_greater->diag || (multi && (!g_lesser->upper || !g_greater->upper)))
     return ctx->fail(RGF_EINVAL, "graph: missing buffers");
+  if (current && !current_obc_ok(obc, g))
+    return ctx->fail(RGF_EINVAL, "current needs lesser/greater OBC blocks on the first and last block");
   if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
   std::unique_lock<std::shared_mutex> lock(g_graph_mutex);
   rgf_graph* gr = new rgf_graph();
@@ -1361,8 +1671,11 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
   }
   const bool have_gr = g_retarded && g_retarded->diag;
   const bool mixed = precision == 1;
-  const size_t wbytes = work_bytes(g, n_energies, true, false, have_gr, mixed, true);
-  const size_t tbytes = g.N * (sizeof(int64_t) + sizeof(int));
+  const bool cur = current != nullptr;
+  const bool small = !mixed && ctx->persistent && small_geometry(g);
+  const size_t wbytes = small ? std::max<size_t>(256, small_work_bytes(g, n_energies, have_gr, cur, ctx->n_cu))
+                              : work_bytes(g, n_energies, true, cur, have_gr, mixed, true);
+  const size_t tbytes = table_bytes(g);
   // An FP64 graph shares a caller-provided context workspace that is large
   // enough (it must then outlive the graph; graphs and solves on the context
   // stay serial on its stream).  Otherwise - and always for the mixed sweep,
@@ -1377,23 +1690,21 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
   }
   rc = ctx->hip(hipMalloc(&gr->tables, tbytes), "graph tables");
   if (rc) { delete gr; return rc; }
-  {
-    std::vector<char> host(tbytes);
-    std::memcpy(host.data(), g.od.data(), g.N * sizeof(int64_t));
-    std::vector<int> sz(g.N);
-    for (int i = 0; i < g.N; ++i) sz[i] = (int)g.bs[i];
-    std::memcpy(host.data() + g.N * sizeof(int64_t), sz.data(), g.N * sizeof(int));
-    rc = ctx->hip(hipMemcpyAsync(gr->tables, host.data(), tbytes, hipMemcpyHostToDevice, ctx->stream),
-                  "graph tables copy");
-    if (!rc) rc = ctx->hip(hipStreamSynchronize(ctx->stream), "graph tables sync");
-    if (rc) { delete gr; return rc; }
-  }
+  // The graph records with its own table memory: geometry tables written
+  // now, the job list of the current by a kernel node of the graph itself.
+  const TableState saved_tables = TableState::save(*ctx);
+  ctx->tables = static_cast<char*>(gr->tables);
+  ctx->tables_cap = tbytes;
+  ctx->tables_fixed = true;
+  rc = ctx->upload_geometry(g);
+  if (!rc) rc = ctx->hip(hipStreamSynchronize(ctx->stream), "graph tables sync");
+  if (rc) { saved_tables.restore(ctx); delete gr; return rc; }
   Work w;
-  carve(g, n_energies, true, false, have_gr, wsp, &w, mixed, true);
+  carve(g, n_energies, true, cur, have_gr, wsp, &w, mixed, true);
   if (mixed) {
     rc = ctx->hip(launch_identity(w.ident, (int)g.bmax, ctx->stream), "identity");
     if (!rc) rc = ctx->hip(hipStreamSynchronize(ctx->stream), "identity sync");
-    if (rc) { delete gr; return rc; }
+    if (rc) { saved_tables.restore(ctx); delete gr; return rc; }
   }
   SolveIO io;
   io.A = view(g, a->diag, a->upper, a->lower, 0);
@@ -1404,14 +1715,10 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
                   : BTV{&g, nullptr, nullptr, nullptr, 0, -1, -1};
   io.GL = view(g, g_lesser->diag, g_lesser->upper, g_lesser->lower, 0);
   io.GG = view(g, g_greater->diag, g_greater->upper, g_greater->lower, 0);
-  io.current = nullptr;
+  io.current = static_cast<double2*>(current);
   // record with the graph's own tables and flags; restore the context afterwards
-  int64_t* saved_offs = ctx->offs_dev;
-  int* saved_sizes = ctx->sizes_dev;
   const bool saved_prof = ctx->profile, saved32 = ctx->prec32;
   const int saved_ref = ctx->refine;
-  ctx->offs_dev = static_cast<int64_t*>(gr->tables);
-  ctx->sizes_dev = reinterpret_cast<int*>(static_cast<char*>(gr->tables) + g.N * sizeof(int64_t));
   ctx->profile = false;
   ctx->prec32 = mixed;
   ctx->refine = mixed ? refine : 0;
@@ -1429,13 +1736,14 @@ int rgf_graph_capture_solve(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_en
   int rc_sweep = RGF_OK;
   if (!rc) {
     rc_sweep = ctx->reset_info();
-    if (!rc_sweep) rc_sweep = sweep_chunk_solve(ctx, g, 0, (int)n_energies, io, w);
+    if (!rc_sweep)
+      rc_sweep = small ? run_small(ctx, {SmallSpec{&g, n_energies, io}}, wsp)
+                       : sweep_chunk_solve(ctx, g, 0, (int)n_energies, io, w);
     rc = ctx->hip(hipStreamEndCapture(cap, &graph), "end capture");
   }
   ctx->stream = saved_stream;
   if (cap) hipStreamDestroy(cap);
-  ctx->offs_dev = saved_offs;
-  ctx->sizes_dev = saved_sizes;
+  saved_tables.restore(ctx);
   ctx->profile = saved_prof;
   ctx->prec32 = saved32;
   ctx->refine = saved_ref;
@@ -1560,21 +1868,8 @@ int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_e
   if (n_energies < 0) return ctx->fail(RGF_EINVAL, "n_energies < 0");
   if (n_energies == 0) return RGF_OK;
   const bool multi = g.N > 1;
-  if (!a || !a->diag || (multi && (!a->upper || !a->lower)))
-    return ctx->fail(RGF_EINVAL, "selected_solve: A needs diag, upper and lower");
-  if (!sigma_lesser || !sigma_greater || !sigma_lesser->diag || !sigma_greater->diag ||
-      (multi && (!sigma_lesser->upper || !sigma_greater->upper)))
-    return ctx->fail(RGF_EINVAL, "selected_solve: Sigma needs diag and upper");
-  if (!g_lesser || !g_greater || !g_lesser->diag || !g_greater->diag ||
-      (multi && (!g_lesser->upper || !g_greater->upper)))
-    return ctx->fail(RGF_EINVAL, "selected_solve: G^< and G^> need diag and upper outputs");
-  if (current) {
-    const int last = g.N - 1;
-    if (!obc || !obc->lesser || !obc->greater || !obc->lesser[0] || !obc->greater[0] ||
-        !obc->lesser[last] || !obc->greater[last])
-      return ctx->fail(RGF_EINVAL,
-                       "current needs lesser/greater OBC blocks on the first and last block");
-  }
+  rc = check_solve_args(ctx, g, a, sigma_lesser, sigma_greater, obc, g_lesser, g_greater, current);
+  if (rc) return rc;
   if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
   const bool have_gr = g_retarded && g_retarded->diag;
   const bool have_off = have_gr && (!multi || (g_retarded->upper && g_retarded->lower));

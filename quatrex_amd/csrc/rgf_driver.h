@@ -48,9 +48,17 @@ struct Ctx {
   std::vector<int> tile_table;    // FP64 products, by block size
   std::vector<int> tile_table32;  // FP32-compute products, by block size
   int* info_dev = nullptr;
+  // context-owned table memory, grown on demand
   void* meta = nullptr;
   size_t meta_bytes = 0;
-  size_t meta_used = 0;
+  // Where the current call's geometry tables and job lists (trace / norm
+  // jobs, boundary tables) go: normally `meta`; a graph's own table memory
+  // while the graph is captured; this device's fixed array in the
+  // allocation-free entry point.  `tables_fixed`: never grown or replaced.
+  char* tables = nullptr;
+  size_t tables_cap = 0;
+  size_t tables_used = 0;
+  bool tables_fixed = false;
   int64_t* offs_dev = nullptr;
   int* sizes_dev = nullptr;
   std::string last_error;
@@ -61,6 +69,10 @@ struct Ctx {
   int refine = 0;
   // one product group per launch (reference op order, no grouping)
   bool unfused = false;
+  // geometries with every block <= kSmallMaxBlock run the persistent on-chip
+  // sweep (one workgroup per energy, intermediates in LDS)
+  bool persistent = true;
+  int n_cu = 256;  // compute units of the device (persistent grid size)
   // run each block inversion on a second stream, overlapped with the
   // independent lesser/greater products of the same forward step
   bool overlap = true;
@@ -93,6 +105,8 @@ struct Ctx {
           hipStream_t on = nullptr);
   int reset_info();
   int upload_geometry(const Geo& g);
+  // appends a host table to `tables` (stream-ordered, no host wait)
+  int put_table(const void* host, size_t bytes, void** dev);
   int64_t chunk_for(const Geo& g, int64_t nE, bool solve, bool current, bool have_gr);
 };
 

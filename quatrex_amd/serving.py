@@ -29,17 +29,29 @@ workload: 97 -> 130 requests/s at saturation and a 6x lower p50 below
 capacity with 16 queues and one stream per slot; 24 or 32 queues lowered the
 saturated throughput again (more kernels in flight than the CUs serve well).
 
+Currents: with ``current=True`` every result also carries the N+1
+Meir-Wingreen layer currents.  The boundary then enters as OBC blocks
+instead of being folded into A: B^R = Sigma_B on the first and last block,
+and the boundary part of the generator's Sigma^> (Sigma^> = Sigma^< -
+(Sigma_B - Sigma_B^H) there) as B^> with B^< = 0, so the Green's functions are
+those of the folded workload and J_0, J_N are the currents into the two
+boundaries.
+
 FP64 batches replay a hipGraph of the whole sweep, captured for every batch
 size of a slot when the server starts (serially, before any batch runs; the
 graphs share the slot's workspace) and replayed on the slot's buffers (A is
 re-assembled in place before each replay), which removes the per-launch host
 cost of small geometries.
 
-Precision: ``mixed`` runs the complex64-product sweep with the FP64 check and
-FP64 fallback above ``fallback_tol``; ``adaptive`` additionally sends a
-geometry straight to FP64 while its recent fallback rate is high (the mixed
-attempt would be wasted), retrying mixed after ``retry_every`` batches, the
-interval doubling after every failed retry (up to 1024 batches).
+Precision: FP64 is the default.  ``mixed`` runs the complex64-product sweep
+with the FP64 check and FP64 fallback above ``fallback_tol``; ``adaptive``
+additionally sends a geometry straight to FP64 while its recent fallback rate
+is high (the mixed attempt would be wasted), retrying mixed after
+``retry_every`` batches, the interval doubling after every failed retry (up
+to 1024 batches).  At a 1e-8 tolerance complex64 products fail the check for
+every energy of these workloads (their G^<> error is 1e-5 to 1e-1 against
+FP64, growing as eta falls), so ``mixed`` then costs the check on top of the
+FP64 solve; the modes are kept for looser tolerances.
 """
 
 from __future__ import annotations
@@ -89,7 +101,7 @@ class Result:
         return self.done - self.arrival
 
 
-def slot_bytes(geometry: Geometry, capacity: int) -> int:
+def slot_bytes(geometry: Geometry, capacity: int, current: bool = False) -> int:
     """HBM bytes of one slot: I/O buffers for ``capacity`` energies + workspace."""
     g = geometry
     per_e = 16 * (3 * g.diag_size + 3 * g.offdiag_size      # A
@@ -97,7 +109,10 @@ def slot_bytes(geometry: Geometry, capacity: int) -> int:
                   + 3 * g.diag_size + 2 * g.offdiag_size     # G^R diag, upper, lower
                   + 2 * (g.diag_size + g.offdiag_size))      # G^<, G^> diag, upper
     per_e += 16 + 8 + 4  # z, residual, flag
-    ws = S.workspace_size(g, capacity, solve=True, current=False, retarded=True)
+    if current:
+        ends = [g.block_sizes[i] for i in W.boundary_blocks(g)]
+        per_e += 16 * (g.n_blocks + 1 + 3 * sum(b * b for b in ends))  # current, OBC blocks
+    ws = S.workspace_size(g, capacity, solve=True, current=current, retarded=True)
     ws += 2 * 16 * g.offdiag_size * capacity + 64 * 8 * capacity + 16 * g.max_block ** 2 + 4096
     return per_e * capacity + ws
 
@@ -127,8 +142,20 @@ class Slot:
             self.GG = S.packed_empty(g, C, device, parts=("d", "u"))
             self.resid = torch.zeros(C, dtype=torch.float64, device=device)
             self.flags = torch.zeros(C, dtype=torch.int32, device=device)
-            nbytes = S.workspace_size(g, C, solve=True) + 2 * 16 * g.offdiag_size * C + 64 * 8 * C \
-                + 16 * g.max_block ** 2 + 4096
+            self.cur = None
+            self.obc = None
+            if server.current:
+                # the boundary as OBC blocks (module notes); Sigma^> without its boundary part
+                self.cur = torch.zeros((C, g.n_blocks + 1), dtype=torch.complex128, device=device)
+                self.obc = ([None] * g.n_blocks, [None] * g.n_blocks, [None] * g.n_blocks)
+                for i in W.boundary_blocks(g):
+                    sb = res.sigma_b[i]
+                    self.obc[0][i] = sb.expand(C, -1, -1).contiguous()
+                    self.obc[1][i] = torch.zeros_like(self.obc[0][i])
+                    self.obc[2][i] = (-(sb - sb.conj().T)).expand(C, -1, -1).contiguous()
+                self.SG = (res.sl_diag.expand(C, -1).contiguous(), res.sg_upper.expand(C, -1).contiguous())
+            nbytes = S.workspace_size(g, C, solve=True, current=server.current) \
+                + 2 * 16 * g.offdiag_size * C + 64 * 8 * C + 16 * g.max_block ** 2 + 4096
             self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=device)
         self.stream.synchronize()
         self.ctx.set_workspace(self.workspace)
@@ -173,9 +200,11 @@ class Slot:
         self.z[:n].copy_(zh, non_blocking=False)
         view = lambda T: tuple(t[:n] if t is not None else None for t in T)  # noqa: E731
         A = view(self.A)
-        S.assemble(g, self.z[:n], res.h, a=A, sigma_b=res.sigma_b, ctx=self.ctx)
+        S.assemble(g, self.z[:n], res.h, a=A, sigma_b=None if self.obc else res.sigma_b, ctx=self.ctx)
         common = dict(g_lesser=view(self.GL), g_greater=view(self.GG), g_retarded=view(self.GR),
                       ctx=self.ctx)
+        if self.obc is not None:
+            common.update(obc=self._obc_view(n), current=self.cur[:n])
         SL, SG = view(self.SL), view(self.SG)
         if precision == "fp64":
             graph = self.graphs.get(n)
@@ -191,6 +220,9 @@ class Slot:
         self.ctx.status()
         return self.flags[:n].cpu().numpy().astype(bool)
 
+    def _obc_view(self, n: int):
+        return tuple([t[:n] if t is not None else None for t in lst] for lst in self.obc)
+
     def prepare_graphs(self) -> None:
         """Capture the FP64 sweep for every batch size 1..capacity (server thread,
         before any batch runs): graphs are never captured while other slots
@@ -198,14 +230,21 @@ class Slot:
         g = self.geometry
         view = lambda T, n: tuple(t[:n] if t is not None else None for t in T)  # noqa: E731
         for n in range(1, self.capacity + 1):
+            extra = {}
+            if self.obc is not None:
+                extra = dict(obc=self._obc_view(n), current=self.cur[:n])
             self.graphs[n] = S.SolveGraph(g, view(self.A, n), view(self.SL, n), view(self.SG, n),
                                           g_lesser=view(self.GL, n), g_greater=view(self.GG, n),
-                                          g_retarded=view(self.GR, n), ctx=self.ctx)
+                                          g_retarded=view(self.GR, n), ctx=self.ctx, **extra)
         self.stream.synchronize()
 
     def outputs(self, k: int):
-        """Device views of energy ``k`` of the last batch: (G^R, G^<, G^>) packed."""
-        return tuple(tuple(t[k] if t is not None else None for t in X) for X in (self.GR, self.GL, self.GG))
+        """Device views of energy ``k`` of the last batch: (G^R, G^<, G^>) packed,
+        plus the layer currents when the server computes them."""
+        outs = tuple(tuple(t[k] if t is not None else None for t in X) for X in (self.GR, self.GL, self.GG))
+        if self.cur is not None:
+            outs = outs + ((self.cur[k],),)
+        return outs
 
     def close(self):
         self.inbox.put(None)
@@ -244,19 +283,22 @@ class Server:
         (cached on disk) and use the fastest in every slot
     overlap : run each slot's block inversions on a second stream (off: one
         stream per slot, see the module notes on hardware queues)
+    current : also compute the layer currents of every request (the boundary
+        then enters as OBC blocks, see the module notes)
     """
 
     def __init__(self, geometries, seeds, budget_bytes: int, capacity: int = 8,
-                 slots_per_geometry=2, precision: str = "adaptive",
+                 slots_per_geometry=2, precision: str = "fp64",
                  fallback_tol: float = 1e-8, refine: int = 1, retry_every: int = 8,
                  keep_outputs: set | None = None, use_graphs: bool = True, autotune: bool = False,
-                 overlap: bool = False, device=None):
+                 overlap: bool = False, current: bool = False, device=None):
         import torch
 
         if precision not in ("fp64", "mixed", "adaptive"):
             raise ValueError(precision)
         self.device = torch.device("cuda", 0) if device is None else torch.device(device)
         self.precision = precision
+        self.current = bool(current)
         self.fallback_tol = fallback_tol
         self.refine = refine
         self.retry_every = retry_every
@@ -279,10 +321,10 @@ class Server:
         for gid, g in enumerate(geometries):
             cap = capacity
             nslots = counts[gid]
-            need = slot_bytes(g, cap) * nslots
+            need = slot_bytes(g, cap, self.current) * nslots
             while cap > 1 and self.used + need > self.budget:
                 cap //= 2
-                need = slot_bytes(g, cap) * nslots
+                need = slot_bytes(g, cap, self.current) * nslots
             if self.used + need > self.budget:
                 raise AdmissionError(
                     f"geometry {gid} ({g.n_blocks} blocks, max block {g.max_block}) needs "

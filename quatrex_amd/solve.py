@@ -258,6 +258,60 @@ def selected_solve(geometry: Geometry, a, sigma_lesser, sigma_greater, *, g_less
     _lib.check(rc, ctx.handle)
 
 
+@dataclass
+class SolveProblem:
+    """One selected solve of a batch (the arguments of :func:`selected_solve`)."""
+
+    geometry: Geometry
+    a: tuple
+    sigma_lesser: tuple
+    sigma_greater: tuple
+    g_lesser: tuple
+    g_greater: tuple
+    g_retarded: tuple | None = None
+    obc: object = None
+    current: object = None
+
+
+def selected_solve_batch(problems, ctx: Context | None = None) -> None:
+    """Several FP64 selected solves in one library call (``rgf_selected_solve_batch_z``):
+    problems whose blocks are all at most 32 share one persistent kernel
+    launch (energy points of every such geometry in one device work queue),
+    the others run one after the other.  Asynchronous on the context stream."""
+    ctx = ctx or default_context()
+    structs = (_lib.SolveProblem_t * max(len(problems), 1))()
+    keep = []
+    for k, p in enumerate(problems):
+        geo = p.geometry
+        E = int(p.a[0].shape[0])
+        gstruct, gkeep = _lib.geometry_struct(geo.block_sizes)
+        obc_lists = _obc_lists(p.obc, geo, E)
+        obc_s, okeep = _obc_struct(obc_lists)
+        cur = _check(p.current, (E, geo.n_blocks + 1), "current") if p.current is not None else None
+        if cur is not None and obc_lists is None:
+            raise TypeError("the current needs lesser and greater OBC blocks at both ends")
+        st = structs[k]
+        st.geom = gstruct
+        st.n_energies = E
+        st.a = _bt(p.a, geo, E, "a")
+        st.sigma_lesser = _bt(p.sigma_lesser, geo, E, "sigma_lesser", (True, True, False))
+        st.sigma_greater = _bt(p.sigma_greater, geo, E, "sigma_greater", (True, True, False))
+        st.g_lesser = _bt(p.g_lesser, geo, E, "g_lesser", (True, True, False), const=False)
+        st.g_greater = _bt(p.g_greater, geo, E, "g_greater", (True, True, False), const=False)
+        if p.g_retarded is not None:
+            st.g_retarded = _bt(p.g_retarded, geo, E, "g_retarded", (True, False, False), const=False)
+        if obc_s:
+            keep.append(obc_s)
+            st.obc = ctypes.pointer(obc_s)
+        st.current = cur
+        keep.extend([gkeep, okeep])
+    rc = _lib.load().rgf_selected_solve_batch_z(ctx.handle, len(problems), structs)
+    del keep
+    if rc == _lib.RGF_EINVAL:
+        raise ValueError(_lib.last_error(ctx.handle))
+    _lib.check(rc, ctx.handle)
+
+
 def selected_inv(geometry: Geometry, a, *, x, obc_retarded=None, ctx: Context | None = None) -> None:
     """Selected inversion on device buffers (asynchronous)."""
     ctx = ctx or default_context()
@@ -335,8 +389,8 @@ class SolveGraph:
     """
 
     def __init__(self, geometry: Geometry, a, sigma_lesser, sigma_greater, *, g_lesser, g_greater,
-                 g_retarded=None, obc=None, ctx: Context | None = None, precision: str = "fp64",
-                 refine: int = 1):
+                 g_retarded=None, obc=None, current=None, ctx: Context | None = None,
+                 precision: str = "fp64", refine: int = 1):
         self.ctx = ctx or default_context()
         geo = geometry
         E = int(a[0].shape[0])
@@ -347,18 +401,23 @@ class SolveGraph:
         GG = _bt(g_greater, geo, E, "g_greater", (True, True, False), const=False)
         GR = (_bt(g_retarded, geo, E, "g_retarded", (True, False, False), const=False)
               if g_retarded is not None else None)
-        obc_s, self._keep = _obc_struct(_obc_lists(obc, geo, E))
+        obc_lists = _obc_lists(obc, geo, E)
+        obc_s, self._keep = _obc_struct(obc_lists)
+        cur = _check(current, (E, geo.n_blocks + 1), "current") if current is not None else None
         gstruct, gkeep = _lib.geometry_struct(geo.block_sizes)
         h = ctypes.c_void_p()
         byref = ctypes.byref
         rc = _lib.load().rgf_graph_capture_solve(
             self.ctx.handle, byref(gstruct), E, byref(A), byref(SL), byref(SG),
             byref(obc_s) if obc_s else None, 1 if precision == "mixed" else 0, int(refine),
-            byref(GR) if GR else None, byref(GL), byref(GG), byref(h))
+            byref(GR) if GR else None, byref(GL), byref(GG), cur, byref(h))
+        if rc == _lib.RGF_EINVAL and current is not None and obc_lists is None:
+            raise TypeError(_lib.last_error(self.ctx.handle))
         _lib.check(rc, self.ctx.handle)
         self._h = h
         # keep the buffers alive as long as the graph references them
-        self._buffers = (a, sigma_lesser, sigma_greater, g_lesser, g_greater, g_retarded, obc)
+        self._buffers = (a, sigma_lesser, sigma_greater, g_lesser, g_greater, g_retarded, obc,
+                         current, obc_lists)
 
     def launch(self) -> None:
         _lib.check(_lib.load().rgf_graph_launch(self._h), self.ctx.handle)

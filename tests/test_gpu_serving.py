@@ -135,3 +135,47 @@ def test_autotuned_uneven_batching_matches_alone(device, tmp_path, monkeypatch):
                     assert err <= 1e-12, (rid, err)
     finally:
         srv.close()
+
+
+def test_serving_returns_currents(device):
+    """The server with currents: the boundary enters as OBC blocks, graphs
+    record the current, and every kept request's G^R and layer currents match
+    the dense oracle (the current by brute force from left-connected dense
+    inverses)."""
+    from oracle.dense import dense_current, dense_solve
+    from quatrex_amd import workloads as W
+    from tests.helpers import block_rel_errors, tridiagonal_mask
+
+    geos = [Geometry((24, 40, 16, 33, 24)), Geometry.uniform(4, 48)]
+    seeds = [61, 62]
+    n = 10
+    reqs = [Request(0.0, k, k % 2, -0.8 + 0.17 * k, 1e-2) for k in range(n)]
+    keep = set(range(n))
+    srv = Server(geos, seeds, budget_bytes=8 << 30, capacity=3, slots_per_geometry=1,
+                 precision="fp64", keep_outputs=keep, current=True, device=device)
+    try:
+        assert all(len(s.graphs) == s.capacity for s in srv.slots)
+        results = srv.run(reqs, realtime=False)
+        assert sorted(r.rid for r in results) == list(range(n))
+        for rid in keep:
+            req = reqs[rid]
+            res = srv.resident[req.geometry]
+            g = res.geometry
+            GR, GL, GG, (cur,) = srv.kept[rid]
+            A = W.system_matrix(res.model, [req.energy], [req.eta], fold_boundary=False)
+            a = g.to_dense(*A)
+            sl = g.to_dense(res.model.sl_diag[None], None, None)
+            ends = W.boundary_blocks(g)
+            obc = [[None] * g.n_blocks for _ in range(3)]
+            for i in ends:
+                sb = W.BOUNDARY_SIGMA * np.eye(g.block_sizes[i], dtype=np.complex128)[None]
+                obc[0][i], obc[1][i], obc[2][i] = sb, 0 * sb, -(sb - np.conj(np.swapaxes(sb, -1, -2)))
+            ref = dense_solve(a, sl, sl, g.block_offsets, obc)
+            got = g.to_dense(*(x[None].cpu().numpy() for x in GR))
+            err = block_rel_errors(got, ref[0] * tridiagonal_mask(list(g.block_sizes)), list(g.block_sizes))
+            assert err <= 1e-10, (rid, err)
+            cd = dense_current(a, sl, sl, g.block_offsets, obc)[0]
+            cerr = np.abs(cur.cpu().numpy() - cd).max() / np.abs(cd).max()
+            assert cerr <= 1e-10, (rid, cerr)
+    finally:
+        srv.close()

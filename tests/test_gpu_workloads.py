@@ -230,7 +230,7 @@ def test_eta_cycle_reduced_n_vs_dense(device):
     mu = m & _block_upper(bs)
     errs = []
     for X, r, mask in zip((GR, GL, GG), ref, (m, mu, mu)):
-        parts = tuple(x.cpu().numpy() for x in X) + ((None,) if len(X) == 2 else ())
+        parts = tuple(x.cpu().numpy() if x is not None else None for x in X)
         xd = g.to_dense(*parts)
         errs.append(block_rel_errors(xd * mask, r * mask, bs))
     assert max(errs) <= TOL, errs
@@ -245,7 +245,7 @@ def test_eta_cycle_48x256_fp64_identities(device):
     g, model, en, etas, A, SL, SG = _eta_case(device, 48, 256, 3, W.DEVICE_SEED_BASE + 3)
     GR, GL, GG = _solve(device, g, A, SL, SG)
     for e in range(3):
-        one = lambda X: tuple(x[e:e + 1] for x in X)  # noqa: E731
+        one = lambda X: tuple(x[e:e + 1] if x is not None else None for x in X)  # noqa: E731
         w = identity_residuals(g, one(A), one(SL), one(SG), one(GR), one(GL), one(GG))
         print(f"48x256 eta={etas[e]:g} FP64 identity residuals (R, <, >):", w)
         assert max(w) <= IDENTITY_TOL, (etas[e], w)
@@ -269,8 +269,8 @@ def test_eta_cycle_48x256_mixed_accuracy_and_fallback(device):
     for e in range(3):
         errs = []
         for X, R in zip(unchecked, ref):
-            num = sum(float(torch.linalg.norm(x[e] - r[e]) ** 2) for x, r in zip(X, R))
-            den = sum(float(torch.linalg.norm(r[e]) ** 2) for r in R)
+            num = sum(float(torch.linalg.norm(x[e] - r[e]) ** 2) for x, r in zip(X, R) if x is not None)
+            den = sum(float(torch.linalg.norm(r[e]) ** 2) for r in R if r is not None)
             errs.append((num / den) ** 0.5)
         print(f"48x256 eta={etas[e]:g} mixed error vs FP64 (R, <, >):", errs)
         br, bl = bound[etas[e]]
@@ -284,7 +284,8 @@ def test_eta_cycle_48x256_mixed_accuracy_and_fallback(device):
     assert flags.cpu().tolist() == [1, 1, 1], resid.cpu().tolist()
     for X, R in zip(checked, ref):
         for x, r in zip(X, R):
-            assert torch.equal(x, r)
+            if x is not None:
+                assert torch.equal(x, r)
     torch.cuda.empty_cache()
 
 
