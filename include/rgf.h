@@ -172,6 +172,26 @@ int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energ
                          const rgf_bt_const* sigma_greater, const rgf_obc* obc,
                          rgf_bt* g_retarded, rgf_bt* g_lesser, rgf_bt* g_greater, void* current);
 
+/* rgf_selected_solve_z without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot call hipMalloc (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * Same arguments, kernels and results as rgf_selected_solve_z; its
+ * workspace and tables live in one fixed array per GPU of
+ * RGF_NOALLOC_SCRATCH_BYTES, held until the process ends.  The energies run
+ * in chunks that fit the array; a geometry whose single energy point does
+ * not fit returns RGF_EUNSUPPORTED with the sizes in rgf_last_error.  The
+ * array keeps the last call's contents, and every call on one GPU shares it:
+ * calls that overlap in time on one GPU (several contexts or streams) race
+ * and must be serialised by the caller.  Profiling and the second-stream
+ * inversion overlap are off inside this call (both would create HIP
+ * objects). */
+#define RGF_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int rgf_selected_solve_noalloc_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                                 const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
+                                 const rgf_bt_const* sigma_greater, const rgf_obc* obc,
+                                 rgf_bt* g_retarded, rgf_bt* g_lesser, rgf_bt* g_greater,
+                                 void* current);
+
 /* Several selected solves in one call, e.g. the energy points of different
  * devices (geometries) that arrived together.  Each problem has the meaning
  * of one rgf_selected_solve_z call (g_retarded.diag NULL: no retarded output;

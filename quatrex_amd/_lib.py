@@ -46,6 +46,7 @@ EXPORTED_SYMBOLS = (
     "rgf_assemble_z",
     "rgf_selected_inv_z",
     "rgf_selected_solve_z",
+    "rgf_selected_solve_noalloc_z",
     "rgf_selected_solve_batch_z",
     "rgf_selected_solve_mixed",
     "rgf_profile_read",
@@ -156,6 +157,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.rgf_selected_solve_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
                                          P(BtConst_t), P(BtConst_t), P(Obc_t), P(Bt_t), P(Bt_t),
                                          P(Bt_t), c_void_p]
+    lib.rgf_selected_solve_noalloc_z.argtypes = lib.rgf_selected_solve_z.argtypes
     lib.rgf_selected_solve_batch_z.argtypes = [c_void_p, c_int32, P(SolveProblem_t)]
     lib.rgf_selected_solve_mixed.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
                                              P(BtConst_t), P(BtConst_t), P(Obc_t),

@@ -1214,6 +1214,41 @@ int solve_chunked(qx::Ctx* ctx, const qx::Geo& g, int64_t n_energies, const rgf_
 // launches shared (launches from several threads still run concurrently).
 static std::shared_mutex g_graph_mutex;
 
+// Workspace and tables of rgf_selected_solve_noalloc_z.  In host code the
+// name `scratch` is a host-side stand-in, never a device address: host code
+// uses only the per-device address that hipGetSymbolAddress returned
+// (Ctx::noalloc_scratch).  Each GPU holds its own copy.
+static __device__ __align__(256) char scratch[RGF_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+std::vector<char*> g_scratch_ptr;  // by device ordinal
+
+// Device address of `scratch` on the current device (looked up once per
+// device; the caller has made `device` current).
+int scratch_address(int device, char** out, std::string* err) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (device < 0) {
+    *err = "negative device ordinal";
+    return RGF_EINVAL;
+  }
+  if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return RGF_EHIP;
+    }
+    g_scratch_ptr[device] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return RGF_OK;
+}
+
+}  // namespace
+
 struct rgf_graph {
   rgf_ctx* ctx = nullptr;
   hipGraph_t graph = nullptr;
@@ -1269,6 +1304,10 @@ int rgf_create(int device, void* stream, rgf_ctx** out) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
     ctx->ws_budget = std::min<size_t>((size_t)16 << 30, free_b / 4);
+  // the allocation-free entry point's array on this device (a failure is
+  // reported by that entry point only)
+  if (scratch_address(device, &ctx->noalloc_scratch, &ctx->noalloc_error) != RGF_OK)
+    ctx->noalloc_scratch = nullptr;
   *out = ctx;
   return RGF_OK;
 }
@@ -1505,6 +1544,73 @@ int rgf_selected_solve_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energ
   }
   return solve_chunked(ctx, g, n_energies, a, sigma_lesser, sigma_greater, obc, g_retarded, g_lesser,
                        g_greater, current);
+  GUARD_END
+}
+
+int rgf_selected_solve_noalloc_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                                 const rgf_bt_const* a, const rgf_bt_const* sigma_lesser,
+                                 const rgf_bt_const* sigma_greater, const rgf_obc* obc,
+                                 rgf_bt* g_retarded, rgf_bt* g_lesser, rgf_bt* g_greater,
+                                 void* current) {
+  if (!ctx) return RGF_EINVAL;
+  GUARD_BEGIN
+  if (!ctx->noalloc_scratch)
+    return ctx->fail(RGF_EHIP, "allocation-free scratch array unavailable: " + ctx->noalloc_error);
+  qx::Geo g;
+  int rc = g.init(geom, &ctx->last_error);
+  if (rc) return rc;
+  if (n_energies < 0) return ctx->fail(RGF_EINVAL, "n_energies < 0");
+  if (n_energies == 0) return RGF_OK;
+  rc = check_solve_args(ctx, g, a, sigma_lesser, sigma_greater, obc, g_lesser, g_greater, current);
+  if (rc) return rc;
+  if (hipSetDevice(ctx->device) != hipSuccess) return ctx->fail(RGF_EHIP, "hipSetDevice");
+  const bool have_gr = g_retarded && g_retarded->diag;
+  const bool cur = current != nullptr;
+  const bool small = ctx->persistent && small_geometry(g);
+  // tables first, then the workspace; the batched sweep runs in chunks that
+  // fit, the persistent sweep needs its whole per-workgroup scratch
+  const size_t total = RGF_NOALLOC_SCRATCH_BYTES;
+  const size_t tbytes = (table_bytes(g) + 255) & ~(size_t)255;
+  const size_t need = small ? small_work_bytes(g, n_energies, have_gr, cur, ctx->n_cu)
+                            : work_bytes(g, 1, true, cur, have_gr);
+  if (tbytes + need > total)
+    return ctx->fail(RGF_EUNSUPPORTED,
+                     "allocation-free solve: this geometry needs " + std::to_string(tbytes + need) +
+                         " bytes of scratch (" + (small ? "persistent sweep" : "one energy point") +
+                         "), the fixed array holds " + std::to_string(total));
+  // point the context at the array for this call, then restore it
+  void* const saved_ws = ctx->ws;
+  const size_t saved_ws_bytes = ctx->ws_bytes;
+  const bool saved_ws_owned = ctx->ws_owned;
+  const TableState saved_tables = TableState::save(*ctx);
+  const bool saved_overlap = ctx->overlap, saved_profile = ctx->profile;
+  ctx->ws = ctx->noalloc_scratch + tbytes;
+  ctx->ws_bytes = total - tbytes;
+  ctx->ws_owned = false;
+  ctx->tables = ctx->noalloc_scratch;
+  ctx->tables_cap = tbytes;
+  ctx->tables_used = 0;
+  ctx->tables_fixed = true;
+  ctx->overlap = false;  // the side stream and its events are created lazily
+  ctx->profile = false;  // profiling creates events
+  rc = ctx->reset_info();
+  if (!rc) {
+    if (small) {
+      rc = run_small(ctx, {SmallSpec{&g, n_energies, solve_io(g, 0, a, sigma_lesser, sigma_greater, obc,
+                                                               g_retarded, g_lesser, g_greater, current)}},
+                     static_cast<char*>(ctx->ws));
+    } else {
+      rc = solve_chunked(ctx, g, n_energies, a, sigma_lesser, sigma_greater, obc, g_retarded, g_lesser,
+                         g_greater, current);
+    }
+  }
+  ctx->ws = saved_ws;
+  ctx->ws_bytes = saved_ws_bytes;
+  ctx->ws_owned = saved_ws_owned;
+  saved_tables.restore(ctx);
+  ctx->overlap = saved_overlap;
+  ctx->profile = saved_profile;
+  return rc;
   GUARD_END
 }
 

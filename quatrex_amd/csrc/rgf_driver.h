@@ -62,6 +62,11 @@ struct Ctx {
   int64_t* offs_dev = nullptr;
   int* sizes_dev = nullptr;
   std::string last_error;
+  // device address of this GPU's fixed scratch array (the allocation-free
+  // entry point), looked up when the context is created; null with the
+  // reason in `noalloc_error` when the lookup failed
+  char* noalloc_scratch = nullptr;
+  std::string noalloc_error;
 
   // mixed precision: sweep products on FP32 matrix cores; `refine` FP64
   // Newton steps per left-connected inverse

@@ -211,14 +211,20 @@ class MixedOptions:
 
 def selected_solve(geometry: Geometry, a, sigma_lesser, sigma_greater, *, g_lesser, g_greater,
                    g_retarded=None, obc=None, current=None, ctx: Context | None = None,
-                   precision: str = "fp64", mixed: MixedOptions | None = None) -> None:
+                   precision: str = "fp64", mixed: MixedOptions | None = None,
+                   no_alloc: bool = False) -> None:
     """Selected solve on device buffers (asynchronous on the context stream).
 
     ``a``: ``(diag, upper, lower)``; ``sigma_*``: ``(diag, upper)``;
     ``g_lesser``/``g_greater``: ``(diag, upper[, lower])`` outputs;
     ``g_retarded``: ``None`` or ``(diag[, upper, lower])``; ``current``:
-    ``None`` or a ``(E, N+1)`` complex128 tensor.
+    ``None`` or a ``(E, N+1)`` complex128 tensor.  ``no_alloc``: the FP64
+    solve through ``rgf_selected_solve_noalloc_z``, which allocates no device
+    memory (its workspace is a fixed per-GPU array, shared by every such call
+    on that GPU: overlapping calls race).
     """
+    if no_alloc and precision != "fp64":
+        raise ValueError("no_alloc is available for precision='fp64' only")
     ctx = ctx or default_context()
     geo = geometry
     E = int(a[0].shape[0])
@@ -236,7 +242,8 @@ def selected_solve(geometry: Geometry, a, sigma_lesser, sigma_greater, *, g_less
     lib = _lib.load()
     byref = ctypes.byref
     if precision == "fp64":
-        rc = lib.rgf_selected_solve_z(ctx.handle, byref(gstruct), E, byref(A), byref(SL),
+        fn = lib.rgf_selected_solve_noalloc_z if no_alloc else lib.rgf_selected_solve_z
+        rc = fn(ctx.handle, byref(gstruct), E, byref(A), byref(SL),
                                       byref(SG), byref(obc_s) if obc_s else None,
                                       byref(GR) if GR else None, byref(GL), byref(GG), cur)
     elif precision == "mixed":
