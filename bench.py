@@ -92,8 +92,11 @@ def parse():
 
 def cpu_baseline(model, energies, eta, budget_s):
     """Time the oracle's numpy restatement on the host (bounded sample)."""
+    import contextlib
+
     from oracle.rgf_ref import BlockTri, selected_solve
     from quatrex_amd import workloads as W
+    from quatrex_amd.layout import Geometry
 
     try:
         from threadpoolctl import threadpool_limits
@@ -122,8 +125,6 @@ def cpu_baseline(model, energies, eta, budget_s):
         selected_solve(A, L, G, return_retarded=True, max_batch_size=n)
         return time.perf_counter() - t0
 
-    import contextlib
-
     limiter = threadpool_limits(limits=cores) if threadpool_limits else contextlib.nullcontext()
     with limiter:
         t1 = run(1)
@@ -136,6 +137,44 @@ def cpu_baseline(model, energies, eta, budget_s):
             ts = run(1)
         single = {"value": 1.0 / ts, "unit": "energy_points/s", "cores": 1,
                   "sample": f"1 energy, 1 BLAS thread, {ts:.1f} s"}
+    # the CPU-runnable configuration: 8 blocks of 32, one energy, repeated
+    # (median over the repeats; 1 BLAS thread, the op sequence is too small
+    # to gain from more)
+    small = None
+    try:
+        g1 = Geometry.uniform(8, 32)
+        m1 = W.make_device(g1, W.DEVICE_SEED_BASE + 1)
+        e1 = W.draw_energies(1)
+        a1 = W.system_matrix(m1, e1, eta)
+        s1 = W.sigmas(m1, 1)
+        bs1 = list(g1.block_sizes)
+
+        def bt1(d, u, l):
+            m = BlockTri(bs1, (1,))
+            for i in range(g1.n_blocks):
+                m._data[(i, i)][...] = g1.diag(d, i)
+                if i + 1 < g1.n_blocks:
+                    m._data[(i, i + 1)][...] = g1.upper(u, i)
+                    m._data[(i + 1, i)][...] = g1.lower(l, i) if l is not None else -np.conj(
+                        np.swapaxes(g1.upper(u, i), -1, -2))
+            return m
+
+        A1, L1, G1 = bt1(*a1), bt1(s1[0], s1[1], None), bt1(s1[2], s1[3], None)
+        reps = []
+        lim = threadpool_limits(limits=1) if threadpool_limits else contextlib.nullcontext()
+        with lim:
+            tstart = time.perf_counter()
+            while len(reps) < 1000 and time.perf_counter() - tstart < min(budget_s, 20.0):
+                t0 = time.perf_counter()
+                selected_solve(A1, L1, G1, return_retarded=True, max_batch_size=1)
+                reps.append(time.perf_counter() - t0)
+        med = float(np.median(reps))
+        small = {"value": round(1.0 / med, 3), "unit": "energy_points/s", "cores": 1,
+                 "median_ms": round(med * 1e3, 4), "repeats": len(reps),
+                 "sample": "8 blocks x 32, one energy, eta=1e-3, median over the repeats "
+                           "(numpy restatement of rgf.py, 1 BLAS thread)"}
+    except Exception as ex:  # pragma: no cover - reported, not fatal
+        small = {"error": repr(ex)}
     cpu_model = None
     try:
         with open("/proc/cpuinfo") as f:
@@ -156,6 +195,7 @@ def cpu_baseline(model, energies, eta, budget_s):
         "sample": f"{n} energies of the {g.n_blocks}x{g.block_sizes[0]} workload (numpy complex128 "
                   f"restatement of rgf.py, {cores} BLAS threads, {tn:.1f} s)",
         "single_core": single,
+        "config_8x32_one_energy": small,
         "cpu_model": cpu_model,
         "blas": blas,
     }
@@ -741,14 +781,15 @@ def host_io_leg(args, dev) -> dict:
 
 
 def serving_leg(args, dev) -> dict:
-    """Continuous batching over 6 resident geometries, Poisson arrivals, adaptive precision."""
+    """Continuous batching over 6 resident geometries, Poisson arrivals, FP64 (the mixed path
+    cannot meet 1e-8 on these workloads, so FP64 is the serving precision)."""
     from quatrex_amd import serving as SV
     from quatrex_amd import workloads as W
 
     geos = SV.draw_geometries(6, W.GEOMETRY_SEED)
     seeds = [W.GEOMETRY_SEED + 1 + k for k in range(len(geos))]
     srv = SV.Server(geos, seeds, budget_bytes=int(args.serving_budget_gib * 2**30), capacity=8,
-                    slots_per_geometry=2, precision="adaptive", fallback_tol=1e-8, refine=1,
+                    slots_per_geometry=2, precision="fp64", fallback_tol=1e-8, refine=1,
                     autotune=True, device=dev)
     try:
         # warm every slot (graph capture, first launches) outside the measured stream
@@ -772,7 +813,9 @@ def serving_leg(args, dev) -> dict:
                     f"Poisson arrivals at {args.serving_rate:g}/s, first {args.serving_requests} "
                     f"requests of the stream, one energy point each, eta=1e-3",
         "offered_rps": args.serving_rate,
-        "precision": "adaptive (mixed with FP64 check, FP64 fallback above 1e-8)",
+        "precision": "fp64",
+        "requests_served": args.serving_requests,
+        "requests_in_config": 20000,
         "hbm_budget_gib": args.serving_budget_gib,
         "slots": len(srv.slots),
         "slot_capacity": sorted({s.capacity for s in srv.slots}),
@@ -1045,8 +1088,6 @@ def main():
             "sample_energies": ns,
             "batched_fp64_speedup": round(eps / eps_plain, 2),
         }
-        if "mixed" in extra:
-            extra["mixed"]["speedup_vs_plain_fp64"] = round(extra["mixed"]["value"] / eps_plain, 2)
         pctx.close()
 
     out = {
@@ -1072,6 +1113,8 @@ def main():
             "precision": "fp64",
             "outputs": "G^R diag+upper+lower, G^< and G^> diag+upper",
             "gemm_tiles": tiles,
+            "serving_requests": f"{args.serving_requests} of the 20000-request Poisson stream "
+                                f"(serving leg; 0 = leg off)",
         },
         "roofline": {
             "bound": "mfma",
@@ -1122,6 +1165,15 @@ def main():
     for key, leg in legs:
         out[key] = run_leg(leg, args, eps)
 
+    # speed-ups against the reference algorithm on vendor libraries (rocBLAS
+    # ZGEMM + torch.linalg.inv at the reference's default batch of 100)
+    tv = out.get("torch_fp64", {}).get("value") if isinstance(out.get("torch_fp64"), dict) else None
+    if tv:
+        out["speedup_vs_torch_fp64"] = round(eps / tv, 3)
+        if "mixed" in out:
+            out["mixed"]["speedup_vs_torch_fp64"] = round(out["mixed"]["value"] / tv, 3)
+            chk = out["mixed"]["with_check_and_fp64_fallback"]
+            chk["speedup_vs_torch_fp64"] = round(chk["value"] / tv, 3)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(model, energies, args.eta, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -146,6 +146,29 @@ int rgf_inv_z(rgf_ctx* ctx, int64_t n, int64_t n_mats, const void* a, void* out)
 int rgf_assemble_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies, const void* z,
                    const rgf_bt_const* h, const void* const* sigma_b, rgf_bt* a);
 
+/* Full system-matrix assembly of the electron solver on the tridiagonal
+ * blocks, replacing ElectronSolver._assemble_system_matrix
+ * (src/quatrex/electron/solver.py:668-709 with :541-665), per energy e:
+ *   M = z_e S - P - (Sigma^R_herm + (Sigma^> - Sigma^<) / 2 + H)
+ * z: device complex128[n_energies] (E + i eta); S: the overlap (NULL: the
+ * identity); P: S_ab (v_a + v_b) / 2 with an overlap, diag(v) without one
+ * (potential NULL: no term); H: one packed tridiagonal Hamiltonian for all
+ * energies, or one per energy (hamiltonian_per_energy = 1, energy strides of
+ * the packed layout); each self-energy NULL (zero) or packed per energy.
+ * Overlap and self-energies need diag, upper and lower. */
+typedef struct rgf_assembly {
+  const void* z;
+  rgf_bt_const hamiltonian;
+  int32_t hamiltonian_per_energy;
+  const rgf_bt_const* overlap;
+  const double* potential;
+  const rgf_bt_const* sigma_retarded_herm;
+  const rgf_bt_const* sigma_lesser;
+  const rgf_bt_const* sigma_greater;
+} rgf_assembly;
+int rgf_assemble_system_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
+                          const rgf_assembly* assembly, rgf_bt* a);
+
 /* Selected inversion X = A^{-1} on the tridiagonal pattern, complex128.
  * Replaces RGF.selected_inv (rgf.py:25-113).  obc_retarded: NULL or a host
  * array of n_blocks device pointers (see rgf_obc).  x->upper / x->lower may be
@@ -235,9 +258,11 @@ int rgf_selected_solve_mixed(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_e
  * timings, -1 for configurations that do not apply.  Allocates temporary
  * device buffers.  No reference counterpart (the autotuner of this build). */
 /* Configurations (index: output tile, threads per workgroup): 0 64x64 (256),
- * 1 32x32 (64), 2 64x32 (128), 3 128x64 (512).  The FP32-compute products
- * are built for 0, 1 and 3. */
-#define RGF_NUM_TILE_CFGS 4
+ * 1 32x32 (64), 2 64x32 (128), 3 128x64 (512); 4 64x64, 5 32x32 and
+ * 6 128x64 with three real MFMAs per complex product (3M: ar br, ai bi and
+ * (ar + ai)(br + bi)) instead of four.  The FP32-compute products are built
+ * for 0, 1 and 3. */
+#define RGF_NUM_TILE_CFGS 7
 int rgf_autotune(rgf_ctx* ctx, int64_t b, int64_t n_energies, int32_t f32, int32_t* best_cfg,
                  float* ms_per_cfg);
 
@@ -323,6 +348,47 @@ int rgf_sparsify_z(rgf_ctx* ctx, const rgf_geometry* geom, int64_t n_energies,
  * (solver.py:468-483): with the overlap blocks s_* NULL, s_00 is the identity
  * and s_01 = s_10 = 0.  a_ij / a_ji may be NULL (not written); outputs may
  * alias their inputs. */
+/* Beyn's contour-integral NEVP solver, replacing Beyn.__call__
+ * (src/qttools/nevp/beyn.py:14-379, SVD projection) for a whole batch:
+ * quadrature of T(z)^-1 = (sum_n z^n a_n)^-1 on the circles |z| = r_o and
+ * |z| = r_i (num_quad_points nodes each, beyn.py:183-238), moments
+ * P_0 = q_0 Y, P_1 = q_1 Y with a d x m_0 probe block Y of uniform [0,1)
+ * real and imaginary parts drawn from `seed` (beyn.py:307-318), thin SVD
+ * of P_0 with the rank cut s > s_max m_0 eps, and the eigenpairs of
+ * U^H P_1 V S^-1 (beyn.py:101-145, 240-282).  m_0 <= 64. */
+typedef struct rgf_beyn_opts {
+  double r_o, r_i;
+  int32_t m_0, num_quad_points;
+  uint64_t seed;
+} rgf_beyn_opts;
+
+/* Mode selection of the spectral OBC solver (spectral.py:62-99 defaults). */
+typedef struct rgf_spectral_opts {
+  double min_decay, max_decay, min_propagation, residual_tolerance, eta_decay;
+  int32_t num_ref_iterations, residual_normalization;
+} rgf_spectral_opts;
+
+/* a_xx: host array of n_coef (3, 5 or 7) device pointers to (n_energies, d, d)
+ * coefficient blocks, lowest order first (n = -(n_coef/2) .. n_coef/2).
+ * ws: device complex128[n_energies][m_0] (0 beyond the rank); vs: device
+ * complex128[n_energies][d][m_0], unit columns within the rank, zero beyond;
+ * n_valid: NULL or device int32[n_energies], the rank per energy. */
+int rgf_beyn_z(rgf_ctx* ctx, int64_t d, int64_t n_energies, int32_t n_coef, const void* const* a_xx,
+               const rgf_beyn_opts* opts, void* ws, void* vs, int32_t* n_valid);
+
+/* Spectral surface Green's function, replacing Spectral.__call__
+ * (src/qttools/boundary_conditions/obc/spectral.py:482-629) with
+ * nevp = Beyn and block_sections = 1: Beyn on (a_ji, a_ii, a_ij), unit-norm
+ * modes, the reflected-mode mask (spectral.py:221-327),
+ * x = inv(a_ii + a_ji (vr / w) inv(vr^H vr) vr^H) (spectral.py:386-436), then
+ * num_ref_iterations steps x = inv(a_ii - a_ji x a_ij).  eigenvalues: NULL or
+ * device complex128[n_energies][m_0]; n_reflected: NULL or device
+ * int32[n_energies], the number of reflected modes. */
+int rgf_spectral_obc_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, const void* a_ii,
+                       const void* a_ij, const void* a_ji, const rgf_beyn_opts* beyn,
+                       const rgf_spectral_opts* opts, void* x_ii, void* eigenvalues,
+                       int32_t* n_reflected);
+
 int rgf_sancho_rubio_z(rgf_ctx* ctx, int64_t b, int64_t n_energies, const void* a_ii,
                        const void* a_ij, const void* a_ji, int32_t max_iterations,
                        double convergence_tol, void* x_ii, int32_t* iterations, double* delta);

@@ -29,8 +29,8 @@ RGF_OPT_OVERLAP = 6
 RGF_OPT_PERSISTENT = 7
 RGF_OPT_TILE_FOR_BLOCK = 1000  # + block size (FP64 products)
 RGF_OPT_TILE_FOR_BLOCK_F32 = 3000  # + block size (FP32-compute products)
-RGF_NUM_TILE_CFGS = 4
-TILE_CFG_NAMES = ("64x64", "32x32", "64x32", "128x64")
+RGF_NUM_TILE_CFGS = 7
+TILE_CFG_NAMES = ("64x64", "32x32", "64x32", "128x64", "64x64-3m", "32x32-3m", "128x64-3m")
 
 # Every function declared in include/rgf.h (checked by the tests).
 EXPORTED_SYMBOLS = (
@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "rgf_set_workspace",
     "rgf_inv_z",
     "rgf_assemble_z",
+    "rgf_assemble_system_z",
     "rgf_selected_inv_z",
     "rgf_selected_solve_z",
     "rgf_selected_solve_noalloc_z",
@@ -57,11 +58,25 @@ EXPORTED_SYMBOLS = (
     "rgf_densify_z",
     "rgf_sparsify_z",
     "rgf_sancho_rubio_z",
+    "rgf_beyn_z",
+    "rgf_spectral_obc_z",
     "rgf_obc_fixed_point_z",
     "rgf_contact_obc_z",
     "rgf_boundary_system_z",
     "rgf_version",
 )
+
+
+class BeynOpts_t(ctypes.Structure):
+    _fields_ = [("r_o", ctypes.c_double), ("r_i", ctypes.c_double), ("m_0", c_int32),
+                ("num_quad_points", c_int32), ("seed", ctypes.c_uint64)]
+
+
+class SpectralOpts_t(ctypes.Structure):
+    _fields_ = [("min_decay", ctypes.c_double), ("max_decay", ctypes.c_double),
+                ("min_propagation", ctypes.c_double), ("residual_tolerance", ctypes.c_double),
+                ("eta_decay", ctypes.c_double), ("num_ref_iterations", c_int32),
+                ("residual_normalization", c_int32)]
 
 
 class Geometry_t(ctypes.Structure):
@@ -74,6 +89,13 @@ class BtConst_t(ctypes.Structure):
 
 class Bt_t(ctypes.Structure):
     _fields_ = [("diag", c_void_p), ("upper", c_void_p), ("lower", c_void_p)]
+
+
+class Assembly_t(ctypes.Structure):
+    _fields_ = [("z", c_void_p), ("hamiltonian", BtConst_t), ("hamiltonian_per_energy", c_int32),
+                ("overlap", POINTER(BtConst_t)), ("potential", c_void_p),
+                ("sigma_retarded_herm", POINTER(BtConst_t)), ("sigma_lesser", POINTER(BtConst_t)),
+                ("sigma_greater", POINTER(BtConst_t))]
 
 
 class Obc_t(ctypes.Structure):
@@ -152,6 +174,7 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.rgf_inv_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p]
     lib.rgf_assemble_z.argtypes = [c_void_p, P(Geometry_t), c_int64, c_void_p, P(BtConst_t),
                                    P(c_void_p), P(Bt_t)]
+    lib.rgf_assemble_system_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(Assembly_t), P(Bt_t)]
     lib.rgf_selected_inv_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t), P(c_void_p),
                                        P(Bt_t)]
     lib.rgf_selected_solve_z.argtypes = [c_void_p, P(Geometry_t), c_int64, P(BtConst_t),
@@ -176,6 +199,10 @@ def _declare(lib: ctypes.CDLL) -> None:
     lib.rgf_sancho_rubio_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
                                        c_int32, ctypes.c_double, c_void_p, P(c_int32),
                                        P(ctypes.c_double)]
+    lib.rgf_beyn_z.argtypes = [c_void_p, c_int64, c_int64, c_int32, P(c_void_p), P(BeynOpts_t),
+                               c_void_p, c_void_p, c_void_p]
+    lib.rgf_spectral_obc_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                                       P(BeynOpts_t), P(SpectralOpts_t), c_void_p, c_void_p, c_void_p]
     lib.rgf_obc_fixed_point_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p, c_void_p]
     lib.rgf_contact_obc_z.argtypes = [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p,

@@ -1,5 +1,5 @@
-"""Open boundary conditions on the device (Sancho-Rubio surface Green's
-function, memoising boundary system, contact self-energies)."""
+"""Open boundary conditions on the device (Sancho-Rubio and spectral surface
+Green's functions, memoising boundary system, contact self-energies)."""
 
 from quatrex_amd.boundary_conditions.obc import (
     OBCSolver,
@@ -8,5 +8,6 @@ from quatrex_amd.boundary_conditions.obc import (
     compute_contact_obc,
     compute_obc,
 )
+from quatrex_amd.boundary_conditions.spectral import Spectral
 
-__all__ = ["OBCSolver", "OBCSystem", "SanchoRubio", "compute_contact_obc", "compute_obc"]
+__all__ = ["OBCSolver", "OBCSystem", "SanchoRubio", "Spectral", "compute_contact_obc", "compute_obc"]

@@ -83,7 +83,11 @@ enum TileCfg : int32_t {
   kTile32x32 = 1,  // 64 threads, 1 wave of 32x32
   kTile64x32 = 2,  // 128 threads, 2 waves of 32x32
   kTile128x64 = 3, // 512 threads, 8 waves of 32x32
-  kNumTileCfgs = 4
+  // the same tiles with three real MFMAs per complex product (3M)
+  kTile64x64_3M = 4,
+  kTile32x32_3M = 5,
+  kTile128x64_3M = 6,
+  kNumTileCfgs = 7
 };
 int tile_m(int cfg);
 int tile_n(int cfg);

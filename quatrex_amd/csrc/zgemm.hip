@@ -3,7 +3,11 @@
 // Each launch carries up to kMaxProblems independent problems
 //   D = sum_t alpha_t op(A_t) op(B_t) + sum_a (c1_a F_a + c2_a F_a^H)
 // over nE energies.  Complex products are four real MFMAs
-// (v_mfma_f64_16x16x4f64): re += ar*br - ai*bi, im += ar*bi + ai*br.
+// (v_mfma_f64_16x16x4f64): re += ar*br - ai*bi, im += ar*bi + ai*br; or,
+// in the 3M configurations, three: P1 = ar*br, P2 = ai*bi and
+// P3 = (ar + ai)(br + bi) accumulate separately and re = P1 - P2,
+// im = P3 - P1 - P2 in the epilogue (25% fewer matrix-core operations, one
+// more accumulator set, two FP64 adds per fragment pair).
 // Tiles of op(A) (TM x TK) and op(B) (TK x TN) are staged through LDS,
 // k-major so that every MFMA fragment read is 16 contiguous complex values;
 // conjugation and alpha are applied while staging, so the inner loop is the
@@ -146,10 +150,14 @@ __device__ inline double4_t mfma_nega(double a, double b, double4_t c) {
 // addends and stores stay complex128 (the mixed-precision sweeps).
 // Occupancy target: four 64x64 workgroups per CU (33 KB LDS each, <= 128
 // VGPRs), measured faster than three.
-template <int TM, int TN, int kRole, bool kF32>
+// k3M: FP64 products by three real MFMAs per complex product (see the file
+// header); the extra accumulator set lowers the 64x64 target to three
+// workgroups per CU.
+template <int TM, int TN, int kRole, bool kF32, bool k3M = false>
 __global__ void __launch_bounds__((TileShape<TM, TN, kF32>::kThreads),
-                                  (TM * TN > 4096 ? 1 : (TM * TN == 4096 ? 4 : 2)))
+                                  (TM * TN > 4096 ? 1 : (TM * TN == 4096 ? (k3M ? 3 : 4) : 2)))
 zgemm_grouped_kernel(const ZGroup g) {
+  static_assert(!(k3M && kF32), "3M products are built for FP64 only");
   using S = TileShape<TM, TN, kF32>;
   using LT = typename std::conditional<kF32, float2, double2>::type;
   using AccT = typename std::conditional<kF32, float4_t, double4_t>::type;
@@ -185,13 +193,15 @@ zgemm_grouped_kernel(const ZGroup g) {
   const int wm = wave / S::kWavesN;
   const int wn = wave % S::kWavesN;
 
-  AccT accr[2][2], acci[2][2];
+  // 4M: accr = re, acci = im.  3M: accr = P1, acci = P2, accs = P3.
+  AccT accr[2][2], acci[2][2], accs[k3M ? 2 : 1][k3M ? 2 : 1];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       accr[i][j] = AccT{0, 0, 0, 0};
       acci[i][j] = AccT{0, 0, 0, 0};
+      if constexpr (k3M) accs[i][j] = AccT{0, 0, 0, 0};
     }
 
   for (int ti = 0; ti < P.nterms; ++ti) {
@@ -222,8 +232,17 @@ zgemm_grouped_kernel(const ZGroup g) {
       for (int j = 0; j < 2; ++j) fb[0][j] = bs[lane >> 4][wn * 32 + j * 16 + (lane & 15)];
 #pragma unroll
       for (int ks = 0; ks < TK / 4; ++ks) {
-        const int c = ks & 1;
-        if (ks + 1 < TK / 4) {
+        // 3M: no fragment prefetch (its registers hold the third accumulator set)
+        const int c = k3M ? 0 : (ks & 1);
+        if constexpr (k3M) {
+          if (ks > 0) {
+            const int kk = ks * 4 + (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) fa[0][i] = as[kk][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) fb[0][j] = bs[kk][wn * 32 + j * 16 + (lane & 15)];
+          }
+        } else if (ks + 1 < TK / 4) {
           const int kk = (ks + 1) * 4 + (lane >> 4);
 #pragma unroll
           for (int i = 0; i < 2; ++i) fa[c ^ 1][i] = as[kk][wm * 32 + i * 16 + (lane & 15)];
@@ -254,6 +273,24 @@ zgemm_grouped_kernel(const ZGroup g) {
 #pragma unroll
             for (int j = 0; j < 2; ++j)
               acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[c][i].y, fb[c][j].x, acci[i][j], 0, 0, 0);
+        } else if constexpr (k3M) {
+          double sa[2], sb[2];
+#pragma unroll
+          for (int i = 0; i < 2; ++i) sa[i] = fa[c][i].x + fa[c][i].y;
+#pragma unroll
+          for (int j = 0; j < 2; ++j) sb[j] = fb[c][j].x + fb[c][j].y;
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) accr[i][j] = mfma(fa[c][i].x, fb[c][j].x, accr[i][j]);
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acci[i][j] = mfma(fa[c][i].y, fb[c][j].y, acci[i][j]);
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) accs[i][j] = mfma(sa[i], sb[j], accs[i][j]);
         } else {
 #pragma unroll
           for (int i = 0; i < 2; ++i)
@@ -307,8 +344,13 @@ zgemm_grouped_kernel(const ZGroup g) {
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int rr = kF32 ? ((lane >> 4) * 4 + r) : (r * 4 + (lane >> 4));
-            Cs[ri + rr][wn * 32 + j * 16 + (lane & 15)] =
-                cmk((double)accr[i][j][r], (double)acci[i][j][r]);
+            if constexpr (k3M) {
+              const double p1 = accr[i][j][r], p2 = acci[i][j][r];
+              Cs[ri + rr][wn * 32 + j * 16 + (lane & 15)] = cmk(p1 - p2, accs[i][j][r] - p1 - p2);
+            } else {
+              Cs[ri + rr][wn * 32 + j * 16 + (lane & 15)] =
+                  cmk((double)accr[i][j][r], (double)acci[i][j][r]);
+            }
           }
       }
     }
@@ -378,7 +420,7 @@ zgemm_grouped_kernel(const ZGroup g) {
   }
 }
 
-template <int TM, int TN, int kRole, bool kF32 = false>
+template <int TM, int TN, int kRole, bool kF32 = false, bool k3M = false>
 hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   ZGroup g = g0;
   int per = 0;  // workgroups per energy (all problems)
@@ -393,7 +435,7 @@ hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   const int64_t total = (int64_t)per * g.nE;
   if (total == 0) return hipSuccess;
   if (total > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole, kF32>), dim3((unsigned)total),
+  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole, kF32, k3M>), dim3((unsigned)total),
                      dim3(TileShape<TM, TN, kF32>::kThreads), 0, stream, g);
   return hipGetLastError();
 }
@@ -404,17 +446,17 @@ static_assert(sizeof(ZGroup) <= 4000, "kernel argument block too large");
 
 int tile_m(int cfg) {
   switch (cfg) {
-    case kTile32x32: return 32;
+    case kTile32x32: case kTile32x32_3M: return 32;
     case kTile64x32: return 64;
-    case kTile128x64: return 128;
+    case kTile128x64: case kTile128x64_3M: return 128;
     default: return 64;
   }
 }
 int tile_n(int cfg) {
   switch (cfg) {
-    case kTile32x32: return 32;
+    case kTile32x32: case kTile32x32_3M: return 32;
     case kTile64x32: return 32;
-    case kTile128x64: return 64;
+    case kTile128x64: case kTile128x64_3M: return 64;
     default: return 64;
   }
 }
@@ -424,6 +466,9 @@ hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream) {
     case kTile32x32: return launch_cfg<32, 32, 0>(g, stream);
     case kTile64x32: return launch_cfg<64, 32, 0>(g, stream);
     case kTile128x64: return launch_cfg<128, 64, 0>(g, stream);
+    case kTile64x64_3M: return launch_cfg<64, 64, 0, false, true>(g, stream);
+    case kTile32x32_3M: return launch_cfg<32, 32, 0, false, true>(g, stream);
+    case kTile128x64_3M: return launch_cfg<128, 64, 0, false, true>(g, stream);
     default: return launch_cfg<64, 64, 0>(g, stream);
   }
 }
@@ -439,6 +484,8 @@ hipError_t launch_zgemm_f32(const ZGroup& g, int cfg, hipStream_t stream) {
 hipError_t launch_zgemm_update(const ZGroup& g, int cfg, hipStream_t stream) {
   switch (cfg) {
     case kTile32x32: return launch_cfg<32, 32, 1>(g, stream);
+    case kTile32x32_3M: return launch_cfg<32, 32, 1, false, true>(g, stream);
+    case kTile64x64_3M: case kTile128x64_3M: return launch_cfg<64, 64, 1, false, true>(g, stream);
     default: return launch_cfg<64, 64, 1>(g, stream);
   }
 }

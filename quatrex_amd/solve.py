@@ -388,6 +388,58 @@ def assemble(geometry: Geometry, z, h, *, a, sigma_b=None, ctx: Context | None =
     _lib.check(rc, ctx.handle)
 
 
+def assemble_system(geometry: Geometry, z, hamiltonian, *, a, overlap=None, potential=None,
+                    sigma_retarded_herm=None, sigma_lesser=None, sigma_greater=None,
+                    ctx: Context | None = None) -> None:
+    """The electron solver's system matrix on the tridiagonal blocks
+    (``src/quatrex/electron/solver.py:668-709``), asynchronous:
+    ``A_e = z_e S - P - (Sigma^R_herm + (Sigma^> - Sigma^<) / 2 + H)``.
+
+    ``z``: ``(E,)`` complex128 (``E + i eta``); ``hamiltonian``: packed
+    ``(diag, upper, lower)``, 1-D (one H for every energy) or ``(E, size)``;
+    ``overlap``: ``None`` (identity) or packed 1-D ``(diag, upper, lower)``;
+    ``potential``: ``None`` or a float64 ``(n,)`` tensor (``S_ab (v_a + v_b) / 2``
+    with an overlap, ``diag(v)`` without); self-energies: ``None`` or packed
+    ``(E, size)`` ``(diag, upper, lower)``; ``a``: packed outputs.
+    """
+    torch = _torch()
+    ctx = ctx or default_context()
+    geo = geometry
+    E = int(z.shape[0])
+    _check(z, (E,), "z")
+    per_energy = hamiltonian[0].dim() == 2
+    sizes = (geo.diag_size, geo.offdiag_size, geo.offdiag_size)
+    names = ("diag", "upper", "lower")
+
+    def shared(tri, name):
+        tri = tuple(tri) + (None,) * (3 - len(tuple(tri)))
+        out = []
+        for t, n, part in zip(tri, sizes, names):
+            if t is None:
+                if part == "diag" or geo.n_blocks > 1:
+                    raise ValueError(f"{name}.{part} is required")
+                out.append(None)
+            else:
+                out.append(_check(t, (n,), f"{name}.{part}"))
+        return _lib.BtConst_t(*out)
+
+    H = _bt(hamiltonian, geo, E, "hamiltonian") if per_energy else shared(hamiltonian, "hamiltonian")
+    S = shared(overlap, "overlap") if overlap is not None else None
+    v = (_check(potential, (geo.n,), "potential", torch.float64) if potential is not None else None)
+    sig = [_bt(t, geo, E, name) if t is not None else None
+           for t, name in ((sigma_retarded_herm, "sigma_retarded_herm"), (sigma_lesser, "sigma_lesser"),
+                           (sigma_greater, "sigma_greater"))]
+    ref = lambda t: ctypes.pointer(t) if t is not None else None  # noqa: E731
+    spec = _lib.Assembly_t(ctypes.c_void_p(z.data_ptr()), H, 1 if per_energy else 0, ref(S), v,
+                           ref(sig[0]), ref(sig[1]), ref(sig[2]))
+    Aout = _bt(a, geo, E, "a", const=False)
+    gstruct, gkeep = _lib.geometry_struct(geo.block_sizes)
+    rc = _lib.load().rgf_assemble_system_z(ctx.handle, ctypes.byref(gstruct), E, ctypes.byref(spec),
+                                           ctypes.byref(Aout))
+    del gkeep
+    _lib.check(rc, ctx.handle)
+
+
 class SolveGraph:
     """A captured selected-solve sweep for fixed device buffers (hipGraph).
 
