@@ -735,10 +735,38 @@ def small_block_leg(args, dev) -> dict:
                      "intensity_flop_per_byte": round(prof["gemm_flops"] / max(prof["gemm_bytes"], 1), 2)},
             "inverse_ms_share": round(prof["inv_ms"] / max(prof["inv_ms"] + prof["gemm_ms"], 1e-9), 4),
         }
+        if b <= 32:
+            # the same solve on the launch-per-product-group sweep, for comparison
+            out[f"{nb}x{b}"]["path"] = "persistent on-chip sweep"
+            pctx = S.Context(dev)
+            pctx.set_option(_lib_opt("RGF_OPT_PERSISTENT"), 0)
+            autotune.tune(pctx, b, n_energies=min(E, 1024))
+
+            def step2():
+                S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG,
+                                 g_retarded=GR, ctx=pctx)
+
+            step2()
+            pctx.status()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                step2()
+            torch.cuda.synchronize()
+            t2 = (time.perf_counter() - t0) / args.steps
+            pctx.status()
+            out[f"{nb}x{b}"]["launch_per_product_energies_per_s"] = round(E / t2, 1)
+            pctx.close()
         del A, sld, sgd, slu, sgu, GR, GL, GG
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
     return out
+
+
+def _lib_opt(name: str) -> int:
+    from quatrex_amd import _lib
+
+    return getattr(_lib, name)
 
 
 def host_io_leg(args, dev) -> dict:
@@ -972,7 +1000,7 @@ def main():
     f_e = flops_per_energy(args.n_blocks, args.block_size)
     gemm_avg_ms = prof["gemm_ms"] / max(prof["gemm_launches"], 1)
     gemm_flops_launch = prof["gemm_flops"] / max(prof["gemm_launches"], 1)
-    achieved = gemm_flops_launch / (gemm_avg_ms * 1e-3) / 1e12
+    achieved = gemm_flops_launch / (gemm_avg_ms * 1e-3) / 1e12 if gemm_avg_ms > 0 else 0.0
     traffic = None
     tf = Path(args.traffic_file)
     if tf.exists():

@@ -20,9 +20,13 @@
 // to fit the nine slots).
 //
 // Product tiles: wave w computes the 16 x 16 output tile (w / 2, w % 2) with
-// v_mfma_f64_16x16x4f64, complex as four real products into four
-// accumulators.  Operand fragments come straight from LDS slots stored
-// row-major with a 33-element row stride (conflict-free column reads).
+// v_mfma_f64_16x16x4f64, complex as three real products (3M: ar br, ai bi,
+// (ar + ai)(br + bi)) into three accumulators.  All operand fragments of a
+// product are read from the LDS slots (row-major, 33-element row stride:
+// conflict-free column reads) before its first MFMA, so the MFMA chain does
+// not wait on LDS between k-steps.  The input blocks of a step are loaded
+// from HBM together (one latency per step, not one per block), and the
+// global addends of the W product are prefetched into registers.
 // Blocks smaller than 32 are zero-padded; tiles that lie outside a
 // product's valid rows / columns skip their MFMAs and the k loop stops at the
 // valid inner size.  Inverted blocks carry an identity pad, which no product
@@ -48,7 +52,7 @@ struct SmallShared {
 };
 
 struct Acc {
-  double4_t rr, ii, ri, ir;
+  double4_t p1, p2, p3;  // ar br, ai bi, (ar + ai)(br + bi)
 };
 
 __device__ inline double4_t mf(double a, double b, double4_t c) {
@@ -57,42 +61,46 @@ __device__ inline double4_t mf(double a, double b, double4_t c) {
 
 __device__ inline Acc acc_zero() {
   Acc a;
-  a.rr = double4_t{0, 0, 0, 0};
-  a.ii = a.rr;
-  a.ri = a.rr;
-  a.ir = a.rr;
+  a.p1 = double4_t{0, 0, 0, 0};
+  a.p2 = a.p1;
+  a.p3 = a.p1;
   return a;
 }
 
 // acc += sgn * op(A) op(B) over the inner index range [0, k) for this wave's
-// tile (tr, tc); op = plain or conjugate transpose (ah / bh).
+// tile (tr, tc); op = plain or conjugate transpose (ah / bh).  Every
+// fragment is loaded first (k-steps beyond the valid range read in-bounds
+// pad and are zeroed by selection), then the MFMAs run back to back.
 template <bool AH, bool BH>
 __device__ inline void mma(Acc& c, const Slot& A, const Slot& B, int k, double sgn, int tr, int tc,
                            int lane) {
   const int l16 = lane & 15, kq = lane >> 4;
   const int row = tr * 16 + l16, col = tc * 16 + l16;
   const int nks = (k + 3) >> 2;
+  double ax[kB / 4], ay[kB / 4], bx[kB / 4], by[kB / 4];
 #pragma unroll
   for (int s = 0; s < kB / 4; ++s) {
-    if (s < nks) {
-      const int kk = s * 4 + kq;
-      const double2 a = AH ? A[kk][row] : A[row][kk];
-      const double2 b = BH ? B[col][kk] : B[kk][col];
-      const double ay = AH ? -a.y : a.y;
-      const double by = BH ? -b.y : b.y;
-      const double ax = sgn * a.x, ayy = sgn * ay;
-      c.rr = mf(ax, b.x, c.rr);
-      c.ii = mf(-ayy, by, c.ii);
-      c.ri = mf(ax, by, c.ri);
-      c.ir = mf(ayy, b.x, c.ir);
-    }
+    const int kk = s * 4 + kq;
+    const double2 a = AH ? A[kk][row] : A[row][kk];
+    const double2 b = BH ? B[col][kk] : B[kk][col];
+    const bool on = s < nks;
+    ax[s] = on ? sgn * a.x : 0.0;
+    ay[s] = on ? (AH ? -sgn * a.y : sgn * a.y) : 0.0;
+    bx[s] = on ? b.x : 0.0;
+    by[s] = on ? (BH ? -b.y : b.y) : 0.0;
+  }
+#pragma unroll
+  for (int s = 0; s < kB / 4; ++s) {
+    c.p1 = mf(ax[s], bx[s], c.p1);
+    c.p2 = mf(ay[s], by[s], c.p2);
+    c.p3 = mf(ax[s] + ay[s], bx[s] + by[s], c.p3);
   }
 }
 
 // Register r of a lane holds tile element (4 r + lane / 16, lane % 16).
 __device__ inline void tile_vals(const Acc& c, double2 (&v)[4]) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r) v[r] = cmk(c.rr[r] + c.ii[r], c.ri[r] + c.ir[r]);
+  for (int r = 0; r < 4; ++r) v[r] = cmk(c.p1[r] - c.p2[r], c.p3[r] - c.p1[r] - c.p2[r]);
 }
 
 // Element of a global block (row-major, ld), zero outside rows x cols.
@@ -108,14 +116,20 @@ struct Add {
   const double2* g;
   int ld, rows, cols;
   double c1, c2;  // real coefficients suffice for every use here
+  const double2* r;  // this thread's four tile elements, prefetched into registers
 };
 __device__ inline Add add_slot(const Slot& s, double c1, double c2 = 0.0) {
-  return Add{&s, nullptr, 0, 0, 0, c1, c2};
+  return Add{&s, nullptr, 0, 0, 0, c1, c2, nullptr};
 }
 __device__ inline Add add_glob(const double2* g, int ld, int rows, int cols, double c1) {
-  return Add{nullptr, g, ld, rows, cols, c1, 0.0};
+  return Add{nullptr, g, ld, rows, cols, c1, 0.0, nullptr};
 }
-__device__ inline Add add_none() { return Add{nullptr, nullptr, 0, 0, 0, 0.0, 0.0}; }
+__device__ inline Add add_regs(const double2* r) { return Add{nullptr, nullptr, 0, 0, 0, 1.0, 0.0, r}; }
+__device__ inline Add add_none() { return Add{nullptr, nullptr, 0, 0, 0, 0.0, 0.0, nullptr}; }
+
+// Element (row, col) of this wave's tile that register r of `lane` holds.
+__device__ inline int tile_row(int tr, int r, int lane) { return tr * 16 + 4 * r + (lane >> 4); }
+__device__ inline int tile_col(int tc, int lane) { return tc * 16 + (lane & 15); }
 
 __device__ inline double2 add_val(const Add& a, int r, int c) {
   double2 out = cmk(0.0, 0.0);
@@ -133,34 +147,78 @@ __device__ inline double2 add_val(const Add& a, int r, int c) {
   return out;
 }
 
-// Zero-filled load of a global block (rows x cols, ld) into a slot, with
+// Zero-filled loads of global blocks (rows x cols, ld) into slots, with
 // c2 * src2 (same shape; optional) added and `pad` on the diagonal beyond the
-// valid square (the identity pad of a block about to be inverted).
+// valid square (the identity pad of a block about to be inverted).  The
+// global loads of all NB blocks are issued before the barrier that protects
+// the slots' previous readers, so the blocks share one memory latency; the
+// caller synchronises before reading the slots.
+struct Blk {
+  Slot* s;
+  const double2* src;
+  int ld, rows, cols;
+  const double2* src2;
+  double c2, pad;
+};
+__device__ inline Blk blk(Slot& s, const double2* src, int ld, int rows, int cols,
+                          const double2* src2 = nullptr, double c2 = 0.0, double pad = 0.0) {
+  return Blk{&s, src, ld, rows, cols, src2, c2, pad};
+}
+template <int NB>
+__device__ inline void load_blks(const Blk (&b)[NB]) {
+  const int tid = threadIdx.x;
+  double2 v[NB][kB * kB / kThreads];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+#pragma unroll
+    for (int q = 0; q < kB * kB / kThreads; ++q) {
+      const int idx = tid + q * kThreads;
+      const int r = idx >> 5, c = idx & 31;
+      double2 x = gld(b[i].src, b[i].ld, b[i].rows, b[i].cols, r, c);
+      if (b[i].src2 && r < b[i].rows && c < b[i].cols) {
+        const double2 w = b[i].src2[(int64_t)r * b[i].ld + c];
+        x = cmk(x.x + b[i].c2 * w.x, x.y + b[i].c2 * w.y);
+      }
+      if (r == c && r >= b[i].rows) x = cmk(b[i].pad, 0.0);
+      v[i][q] = x;
+    }
+  }
+  __syncthreads();  // the slots' previous content may still be read
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+#pragma unroll
+    for (int q = 0; q < kB * kB / kThreads; ++q) {
+      const int idx = tid + q * kThreads;
+      (*b[i].s)[idx >> 5][idx & 31] = v[i][q];
+    }
+  }
+}
 __device__ inline void load_blk(Slot& s, const double2* src, int ld, int rows, int cols,
                                 const double2* src2 = nullptr, double c2 = 0.0, double pad = 0.0) {
-  const int tid = threadIdx.x;
-  __syncthreads();  // the slot's previous content may still be read
-#pragma unroll
-  for (int q = 0; q < kB * kB / kThreads; ++q) {
-    const int idx = tid + q * kThreads;
-    const int r = idx >> 5, c = idx & 31;
-    double2 v = gld(src, ld, rows, cols, r, c);
-    if (src2 && r < rows && c < cols) {
-      const double2 w = src2[(int64_t)r * ld + c];
-      v = cmk(v.x + c2 * w.x, v.y + c2 * w.y);
-    }
-    if (r == c && r >= rows) v = cmk(pad, 0.0);
-    s[r][c] = v;
-  }
+  const Blk b[1] = {blk(s, src, ld, rows, cols, src2, c2, pad)};
+  load_blks<1>(b);
 }
 
 // Global write of a slot's valid part.  mode 0: dst = scale * s;
 // 1: dst[r][c] = scale * conj(s[c][r]) (dst is cols x rows of the slot);
 // 2: dst = (s - s^H) / 2 (square).
+// Mode 0 reads, per thread, exactly the elements the thread wrote as a
+// product epilogue (wave tile mapping), so a product may overwrite the slot
+// right after without a barrier; modes 1 and 2 read across threads.
 __device__ inline void store_blk(double2* dst, int ld, int rows, int cols, const Slot& s, int mode,
                                  double scale = 1.0) {
   if (!dst) return;
   const int tid = threadIdx.x;
+  if (mode == 0) {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int tr = wave >> 1, tc = wave & 1, c = tile_col(tc, lane);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = tile_row(tr, q, lane);
+      if (r < rows && c < cols) dst[(int64_t)r * ld + c] = cmk(scale * s[r][c].x, scale * s[r][c].y);
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < kB * kB / kThreads; ++q) {
     const int idx = tid + q * kThreads;
@@ -305,11 +363,15 @@ namespace {
 // D = sum of products (the `terms` callback) + addends for the whole 32 x 32
 // slot `dst`; the tile of wave w is (w / 2, w % 2) and tiles outside the
 // valid m x n skip their MFMAs.  `partial`: the value after the first
-// `partial_after` addends also goes there (padded 32 x 32 layout).  The
-// destination may be an operand or addend: writes wait for every read.
+// `partial_after` addends also goes there (padded 32 x 32 layout).  Each
+// thread writes the elements of D it computed; `sync_first` waits for every
+// wave before those writes, which is needed when D is an operand, a
+// transposed addend, or a slot other threads read since the last barrier
+// (a transposing or projecting store).  The trailing barrier publishes D.
 template <typename F>
 __device__ inline void product(SmallShared& sh, int dst, int m, int n, F&& terms, Add a0, Add a1,
-                               Add a2, double2* partial = nullptr, int partial_after = 0) {
+                               Add a2, double2* partial = nullptr, int partial_after = 0,
+                               bool sync_first = false) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int tr = wave >> 1, tc = wave & 1;
   Acc acc = acc_zero();
@@ -317,28 +379,25 @@ __device__ inline void product(SmallShared& sh, int dst, int m, int n, F&& terms
   double2 v[4];
   tile_vals(acc, v);
   const Add adds[3] = {a0, a1, a2};
-  const int col = tc * 16 + (lane & 15);
+  const int col = tile_col(tc, lane);
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (partial && a == partial_after) {
       // the value before the remaining addends (Sigma~ of the current),
       // padded 32 x 32 layout
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = tr * 16 + 4 * r + (lane >> 4);
-        partial[row * kB + col] = v[r];
-      }
+      for (int r = 0; r < 4; ++r) partial[tile_row(tr, r, lane) * kB + col] = v[r];
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int row = tr * 16 + 4 * r + (lane >> 4);
-      v[r] = cadd(v[r], add_val(adds[a], row, col));
+      if (adds[a].r) v[r] = cadd(v[r], adds[a].r[r]);
+      else v[r] = cadd(v[r], add_val(adds[a], tile_row(tr, r, lane), col));
     }
   }
-  __syncthreads();  // every wave has read its operands and addends
+  if (sync_first) __syncthreads();
   Slot& D = sh.s[dst];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) D[tr * 16 + 4 * r + (lane >> 4)][col] = v[r];
+  for (int r = 0; r < 4; ++r) D[tile_row(tr, r, lane)][col] = v[r];
   __syncthreads();
 }
 
@@ -383,19 +442,21 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
     {
       const int b = bs[0];
       cg = S.take();
-      load_blk(sh.s[cg], D_(P.Ad, 0), b, b, b, OB(P.obr, 0), -1.0, 1.0);
+      const int ws[2] = {S.take(), S.take()};
+      {
+        const Blk blks[3] = {blk(sh.s[cg], D_(P.Ad, 0), b, b, b, OB(P.obr, 0), -1.0, 1.0),
+                             blk(sh.s[ws[0]], D_(P.SLd, 0), b, b, b, OB(P.obl, 0), 1.0),
+                             blk(sh.s[ws[1]], D_(P.SGd, 0), b, b, b, OB(P.obg, 0), 1.0)};
+        load_blks<3>(blks);
+      }
       __syncthreads();
       invert(sh, sh.s[cg], b, L.info, 1, eg);
       store_blk(Gd(0), Gld(0), b, b, sh.s[cg], 0);
-      const double2* SL[2] = {D_(P.SLd, 0), D_(P.SGd, 0)};
-      const double2* const* BO[2] = {P.obl, P.obg};
       double2* OUT[2] = {Dw(P.GLd, 0), Dw(P.GGd, 0)};
       int res[2];
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        const int w = S.take();
-        load_blk(sh.s[w], SL[k], b, b, b, OB(BO[k], 0), 1.0);
-        __syncthreads();
+        const int w = ws[k];
         const int v = S.take();
         product(sh, v, b, b, [&](Acc& a, int tr, int tc, int lane) {
           mma<false, false>(a, sh.s[cg], sh.s[w], b, 1.0, tr, tc, lane);
@@ -415,20 +476,38 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
     for (int j = 1; j < N; ++j) {
       const int i = j - 1;
       const int bi = bs[i], bj = bs[j];
-      const int a = S.take();
-      load_blk(sh.s[a], U_(P.Al, i), bi, bj, bi);  // A_ji (bj x bi)
+      // the step's input blocks in one load: A_ji, A_ij, A_jj - B^R_j,
+      // Sigma^<_ij, Sigma^>_ij; the W addends Sigma^<>_jj + B^<>_j of this
+      // thread's tile elements go to registers
+      const int a = S.take(), t = S.take(), s = S.take(), sgl = S.take(), sgg = S.take();
+      {
+        const Blk b[5] = {blk(sh.s[a], U_(P.Al, i), bi, bj, bi),              // A_ji (bj x bi)
+                          blk(sh.s[t], U_(P.Au, i), bj, bi, bj),              // A_ij (bi x bj)
+                          blk(sh.s[s], D_(P.Ad, j), bj, bj, bj, OB(P.obr, j), -1.0, 1.0),
+                          blk(sh.s[sgl], U_(P.SLu, i), bj, bi, bj),           // Sigma^<_ij
+                          blk(sh.s[sgg], U_(P.SGu, i), bj, bi, bj)};          // Sigma^>_ij
+        load_blks<5>(b);
+      }
+      double2 wadd[2][4];
+      {
+        const int lane = tid & 63, wave = tid >> 6, c = tile_col(wave & 1, lane);
+        const double2* SD[2] = {D_(P.SLd, j), D_(P.SGd, j)};
+        const double2* BO[2] = {OB(P.obl, j), OB(P.obg, j)};
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int rr = tile_row(wave >> 1, r, lane);
+            wadd[k][r] = cadd(gld(SD[k], bj, bj, bj, rr, c), gld(BO[k], bj, bj, bj, rr, c));
+          }
+      }
       __syncthreads();
       const int pp = S.take();
       product(sh, pp, bj, bi, [&](Acc& ac, int tr, int tc, int lane) {
         mma<false, false>(ac, sh.s[a], sh.s[cg], bi, 1.0, tr, tc, lane);
       }, add_none(), add_none(), add_none());
       S.give(cg);
-      // S = A_jj - B^R_j - P A_ij
-      const int t = S.take();
-      const int s = S.take();
-      load_blk(sh.s[t], U_(P.Au, i), bj, bi, bj);  // A_ij (bi x bj)
-      load_blk(sh.s[s], D_(P.Ad, j), bj, bj, bj, OB(P.obr, j), -1.0, 1.0);
-      __syncthreads();
+      // S = A_jj - B^R_j - P A_ij (in place: each thread reads its own elements)
       product(sh, s, bj, bj, [&](Acc& ac, int tr, int tc, int lane) {
         mma<false, false>(ac, sh.s[pp], sh.s[t], bi, -1.0, tr, tc, lane);
       }, add_slot(sh.s[s], 1.0), add_none(), add_none());
@@ -437,9 +516,7 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
       store_blk(Gd(j), Gld(j), bj, bj, sh.s[s], 0);
       // lesser, then greater: Y = A_ji g^<>_i ; Q = P Sigma_ij ;
       // W = Y A_ji^H + Q^H - Q [= Sigma~] + Sigma_jj + B_j ; g^<>_j = (g_j W) g_j^H
-      const double2* SU[2] = {U_(P.SLu, i), U_(P.SGu, i)};
-      const double2* SD[2] = {D_(P.SLd, j), D_(P.SGd, j)};
-      const double2* const* BO[2] = {P.obl, P.obg};
+      const int SU[2] = {sgl, sgg};
       double2* OUT[2] = {Dw(P.GLd, j), Dw(P.GGd, j)};
       int carried[2] = {cl, cgg};
       int res[2];
@@ -450,19 +527,15 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
           mma<false, false>(ac, sh.s[a], sh.s[carried[k]], bi, 1.0, tr, tc, lane);
         }, add_none(), add_none(), add_none());
         S.give(carried[k]);
-        const int t2 = S.take();
-        load_blk(sh.s[t2], SU[k], bj, bi, bj);  // Sigma_ij (bi x bj)
-        __syncthreads();
         const int q = S.take();
         product(sh, q, bj, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[pp], sh.s[t2], bi, 1.0, tr, tc, lane);
+          mma<false, false>(ac, sh.s[pp], sh.s[SU[k]], bi, 1.0, tr, tc, lane);
         }, add_none(), add_none(), add_none());
-        S.give(t2);
+        S.give(SU[k]);
         const int w = S.take();
         product(sh, w, bj, bj, [&](Acc& ac, int tr, int tc, int lane) {
           mma<false, true>(ac, sh.s[y], sh.s[a], bi, 1.0, tr, tc, lane);
-        }, add_slot(sh.s[q], -1.0, 1.0), add_glob(SD[k], bj, bj, bj, 1.0),
-            add_glob(OB(BO[k], j), bj, bj, bj, 1.0), cur ? SigT(k, j) : nullptr, 1);
+        }, add_slot(sh.s[q], -1.0, 1.0), add_regs(wadd[k]), add_none(), cur ? SigT(k, j) : nullptr, 1);
         S.give(y);
         S.give(q);
         const int v = S.take();
@@ -517,24 +590,29 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
     for (int i = N - 2; i >= 0; --i) {
       const int j = i + 1;
       const int bi = bs[i], bj = bs[j];
-      const int X = S.take();
-      load_blk(sh.s[X], Gd(j), Gld(j), bj, bj);  // G_jj (retarded, full)
-      const int g = S.take();
-      load_blk(sh.s[g], Gd(i), Gld(i), bi, bi);  // g_i (left-connected)
+      // the step's input blocks in one load: G_jj (retarded, full), g_i
+      // (left-connected), Sigma^<_ij, Sigma^>_ij, A_ij, A_ji
+      const int X = S.take(), g = S.take(), sgl = S.take(), sgg = S.take(), aij = S.take(),
+                aji = S.take();
+      {
+        const Blk b[6] = {blk(sh.s[X], Gd(j), Gld(j), bj, bj), blk(sh.s[g], Gd(i), Gld(i), bi, bi),
+                          blk(sh.s[sgl], U_(P.SLu, i), bj, bi, bj),
+                          blk(sh.s[sgg], U_(P.SGu, i), bj, bi, bj),
+                          blk(sh.s[aij], U_(P.Au, i), bj, bi, bj),
+                          blk(sh.s[aji], U_(P.Al, i), bi, bj, bi)};
+        load_blks<6>(b);
+      }
       __syncthreads();
       // B^<> = (g_i Sigma^<>_ij) G_jj^H
       int Bk[2];
-      const double2* SU[2] = {U_(P.SLu, i), U_(P.SGu, i)};
+      const int SU[2] = {sgl, sgg};
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
-        const int t = S.take();
-        load_blk(sh.s[t], SU[k], bj, bi, bj);
-        __syncthreads();
         const int z = S.take();
         product(sh, z, bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[g], sh.s[t], bi, 1.0, tr, tc, lane);
+          mma<false, false>(ac, sh.s[g], sh.s[SU[k]], bi, 1.0, tr, tc, lane);
         }, add_none(), add_none(), add_none());
-        S.give(t);
+        S.give(SU[k]);
         Bk[k] = S.take();
         product(sh, Bk[k], bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
           mma<false, true>(ac, sh.s[z], sh.s[X], bj, 1.0, tr, tc, lane);
@@ -542,19 +620,17 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
         S.give(z);
       }
       // U = g_i A_ij ; R = G_jj A_ji
-      int t = S.take();
-      load_blk(sh.s[t], U_(P.Au, i), bj, bi, bj);
-      __syncthreads();
       const int u = S.take();
       product(sh, u, bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
-        mma<false, false>(ac, sh.s[g], sh.s[t], bi, 1.0, tr, tc, lane);
+        mma<false, false>(ac, sh.s[g], sh.s[aij], bi, 1.0, tr, tc, lane);
       }, add_none(), add_none(), add_none());
-      load_blk(sh.s[t], U_(P.Al, i), bi, bj, bi);
-      __syncthreads();
+      S.give(aij);
       const int rr = S.take();
       product(sh, rr, bj, bi, [&](Acc& ac, int tr, int tc, int lane) {
-        mma<false, false>(ac, sh.s[X], sh.s[t], bj, 1.0, tr, tc, lane);
+        mma<false, false>(ac, sh.s[X], sh.s[aji], bj, 1.0, tr, tc, lane);
       }, add_none(), add_none(), add_none());
+      S.give(aji);
+      const int t = S.take();
       // G^R_ij = -U G_jj (t is free now)
       if (P.GRu) {
         product(sh, t, bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
@@ -605,11 +681,11 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
         }, add_slot(sh.s[Bk[k]], 1.0), add_slot(sh.s[t2], -1.0), add_none());
         store_blk(OU[k], bj, bi, bj, sh.s[st], 0);
         store_blk(OL[k], bi, bj, bi, sh.s[st], 1, -1.0);
-        // T1 = K g^<>_i - B U^H  (reuses st)
+        // T1 = K g^<>_i - B U^H  (reuses st, which the transposing store read)
         product(sh, st, bi, bi, [&](Acc& ac, int tr, int tc, int lane) {
           mma<false, false>(ac, sh.s[kk], sh.s[gl], bi, 1.0, tr, tc, lane);
           mma<false, true>(ac, sh.s[Bk[k]], sh.s[u], bj, -1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
+        }, add_none(), add_none(), add_none(), nullptr, 0, true);
         S.give(Bk[k]);
         // G^<>_ii = g^<>_i + T2 U^H + T1 - T1^H  (in place over g^<>_i)
         product(sh, gl, bi, bi, [&](Acc& ac, int tr, int tc, int lane) {
