@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--no-autotune", action="store_true", help="default 64x64 GEMM tiles")
+    ap.add_argument("--tile", default=None,
+                    help="FP64 GEMM tile configuration of the main leg (skips its autotuning)")
     ap.add_argument("--torch-sample", type=int, default=100,
                     help="energies of the torch (rocBLAS/rocSOLVER) reference-algorithm leg (0 = skip)")
     ap.add_argument("--plain-sample", type=int, default=8,
@@ -967,14 +969,19 @@ def main():
     if args.chunk:
         ctx.set_option(1, args.chunk)
     # GEMM tile per block size from the autotuner (timed on this device, cached)
+    from quatrex_amd import _lib
     from quatrex_amd import autotune as AT
 
     tiles = {"f64": "64x64", "f32": "64x64"}
-    if not args.no_autotune:
-        tiles["f64"] = AT.tune(ctx, args.block_size, n_energies=64)["best_name"]
-        if not args.no_mixed:
-            tiles["f32"] = AT.tune(ctx, args.block_size, n_energies=64, f32=True)["best_name"]
-    tile_m, tile_n = tiles["f64"].split("x")
+    if args.tile:
+        tiles["f64"] = args.tile
+        ctx.set_option(_lib.RGF_OPT_TILE_FOR_BLOCK + args.block_size, _lib.TILE_CFG_NAMES.index(args.tile))
+    elif not args.no_autotune:
+        tiles["f64"] = AT.tune(ctx, args.block_size, n_energies=args.energies)["best_name"]
+    if not args.no_mixed and not args.no_autotune:
+        tiles["f32"] = AT.tune(ctx, args.block_size, n_energies=args.energies, f32=True)["best_name"]
+    tile_m, tile_n = tiles["f64"].removesuffix("-3m").split("x")
+    tile_3m = tiles["f64"].endswith("-3m")
 
     def step():
         S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
@@ -1146,7 +1153,8 @@ def main():
         },
         "roofline": {
             "bound": "mfma",
-            "kernel": f"zgemm_grouped_kernel<{tile_m},{tile_n},0> (sweep products)",
+            "kernel": f"zgemm_grouped_kernel<{tile_m}, {tile_n}, 0, false, {str(tile_3m).lower()}> "
+                      f"(sweep products)",
             "achieved": round(achieved, 3),
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -1155,6 +1163,9 @@ def main():
             "avg_launch_ms": round(gemm_avg_ms, 4),
             "flops_per_launch": gemm_flops_launch,
             "launches": prof["gemm_launches"],
+            # 3M tiles execute 6 of the 8 m n k real flops of a complex product
+            # on the matrix cores: their busy fraction is 3/4 of frac
+            "mfma_executed_frac": round(achieved / FP64_PEAK_TFLOPS * (0.75 if tile_3m else 1.0), 4),
         },
         "sweep_roofline": {
             "flops_per_energy": f_e,

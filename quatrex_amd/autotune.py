@@ -37,14 +37,29 @@ def _load(path: Path) -> dict:
         return {}
 
 
-def tune(ctx, block_size: int, n_energies: int = 64, f32: bool = False, path: Path | None = None,
-         force: bool = False) -> dict:
-    """Best tile configuration for ``block_size`` (timed or from the cache)."""
+def default_energies(block_size: int) -> int:
+    """Energies per timed product: enough that one operand stack (b^2 E
+    complex128) exceeds the 256 MB Infinity Cache, as in a production sweep,
+    within [64, 1024].  The ranking of the tiles changes with it: at b = 256
+    the 128x64 3M tile wins with 64 energies (operands cache-resident) and the
+    64x64 3M tile with 256-512."""
+    b = max(1, int(block_size))
+    return int(min(1024, max(64, -(-(256 << 20) // (16 * b * b)))))
+
+
+def tune(ctx, block_size: int, n_energies: int | None = None, f32: bool = False,
+         path: Path | None = None, force: bool = False) -> dict:
+    """Best tile configuration for ``block_size`` (timed or from the cache;
+    a cached entry timed at another energy count is timed again)."""
     path = path or cache_path()
     db = _load(path)
     dev = _device_key(ctx)
     key = f"{'f32' if f32 else 'f64'}:{int(block_size)}"
+    if n_energies is None:
+        n_energies = default_energies(block_size)
     entry = db.get(dev, {}).get(key)
+    if entry is not None and entry.get("n_energies") != int(n_energies):
+        entry = None
     if entry is None or force:
         best = ctypes.c_int32(-1)
         ms = (ctypes.c_float * _lib.RGF_NUM_TILE_CFGS)()

@@ -30,11 +30,16 @@ constexpr int PAD = 1;
 // instead of 16.9 KB) does not cap the number of workgroups per CU below what
 // its registers allow (measured: equal at b = 32, +13-34% at b = 48); FP32
 // products, whose MFMAs take half as long, lose more to the extra barriers
-// than they gain and keep the deeper stage.
-template <int TM, int TN, bool kF32 = false>
+// than they gain and keep the deeper stage.  kTKo overrides the depth: the
+// 3M 128x64 tile (one workgroup per CU by its registers either way) stages
+// k = 16, so that each stage's MFMAs cover the next stage's global-load
+// latency and the barrier count halves (measured 2-4% faster than k = 8 at
+// b = 128-512; the same depth made the 4M 128x64 and the 3M 64x64 tiles
+// slower: 99 KB and 66 KB of LDS cost them workgroups per CU).
+template <int TM, int TN, bool kF32 = false, int kTKo = 0>
 struct TileShape {
   static constexpr bool kSmall = TM == 32 && TN == 32 && !kF32;
-  static constexpr int kTK = kSmall ? 4 : 8;
+  static constexpr int kTK = kTKo ? kTKo : (kSmall ? 4 : 8);
   static constexpr int kSlab = kSmall ? 16 : 32;
   static constexpr int kWavesM = TM / 32;
   static constexpr int kWavesN = TN / 32;
@@ -43,10 +48,10 @@ struct TileShape {
   static constexpr int kBPer = kTK * TN / kThreads;
 };
 
-template <int TM, int TN, bool F32>
+template <int TM, int TN, bool F32, int TKo>
 __device__ inline void load_a(const ZTerm& t, int e, int m, int m0, int k0, int tid,
-                              double2 (&ra)[TileShape<TM, TN, F32>::kAPer]) {
-  using S = TileShape<TM, TN, F32>;
+                              double2 (&ra)[TileShape<TM, TN, F32, TKo>::kAPer]) {
+  using S = TileShape<TM, TN, F32, TKo>;
   const double2* A = t.A + (int64_t)e * t.sA;
   const double2 zero = cmk(0.0, 0.0);
 #pragma unroll
@@ -67,11 +72,11 @@ __device__ inline void load_a(const ZTerm& t, int e, int m, int m0, int k0, int 
 __device__ inline void put(double2* p, double2 v) { *p = v; }
 __device__ inline void put(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
 
-template <int TM, int TN, bool F32, typename LT>
+template <int TM, int TN, bool F32, int TKo, typename LT>
 __device__ inline void store_a(const ZTerm& t, int tid,
-                               const double2 (&ra)[TileShape<TM, TN, F32>::kAPer],
+                               const double2 (&ra)[TileShape<TM, TN, F32, TKo>::kAPer],
                                LT (*As)[TM + PAD]) {
-  using S = TileShape<TM, TN, F32>;
+  using S = TileShape<TM, TN, F32, TKo>;
   const bool cj = (t.opA == kOpC);
 #pragma unroll
   for (int i = 0; i < S::kAPer; ++i) {
@@ -85,10 +90,10 @@ __device__ inline void store_a(const ZTerm& t, int tid,
   }
 }
 
-template <int TM, int TN, bool F32>
+template <int TM, int TN, bool F32, int TKo>
 __device__ inline void load_b(const ZTerm& t, int e, int n, int n0, int k0, int tid,
-                              double2 (&rb)[TileShape<TM, TN, F32>::kBPer]) {
-  using S = TileShape<TM, TN, F32>;
+                              double2 (&rb)[TileShape<TM, TN, F32, TKo>::kBPer]) {
+  using S = TileShape<TM, TN, F32, TKo>;
   const double2* B = t.B + (int64_t)e * t.sB;
   const double2 zero = cmk(0.0, 0.0);
 #pragma unroll
@@ -106,11 +111,11 @@ __device__ inline void load_b(const ZTerm& t, int e, int n, int n0, int k0, int 
   }
 }
 
-template <int TM, int TN, bool F32, typename LT>
+template <int TM, int TN, bool F32, int TKo, typename LT>
 __device__ inline void store_b(const ZTerm& t, int tid,
-                               const double2 (&rb)[TileShape<TM, TN, F32>::kBPer],
+                               const double2 (&rb)[TileShape<TM, TN, F32, TKo>::kBPer],
                                LT (*Bs)[TN + PAD]) {
-  using S = TileShape<TM, TN, F32>;
+  using S = TileShape<TM, TN, F32, TKo>;
   const bool cj = (t.opB == kOpC);
 #pragma unroll
   for (int i = 0; i < S::kBPer; ++i) {
@@ -124,9 +129,9 @@ __device__ inline void store_b(const ZTerm& t, int tid,
   }
 }
 
-template <int TM, int TN, typename LT>
+template <int TM, int TN, typename LT, int TKo>
 struct Smem {
-  using S = TileShape<TM, TN, std::is_same<LT, float2>::value>;
+  using S = TileShape<TM, TN, std::is_same<LT, float2>::value, TKo>;
   static constexpr int kA = S::kTK * (TM + PAD);  // one LDS stage of op(A), k-major (LT elements)
   static constexpr int kB = S::kTK * (TN + PAD);  // one LDS stage of op(B), k-major
   static constexpr int kMainBytes = (2 * kA + 2 * kB) * (int)sizeof(LT);
@@ -153,15 +158,15 @@ __device__ inline double4_t mfma_nega(double a, double b, double4_t c) {
 // k3M: FP64 products by three real MFMAs per complex product (see the file
 // header); the extra accumulator set lowers the 64x64 target to three
 // workgroups per CU.
-template <int TM, int TN, int kRole, bool kF32, bool k3M = false>
-__global__ void __launch_bounds__((TileShape<TM, TN, kF32>::kThreads),
+template <int TM, int TN, int kRole, bool kF32, bool k3M = false, int kTKo = 0>
+__global__ void __launch_bounds__((TileShape<TM, TN, kF32, kTKo>::kThreads),
                                   (TM * TN > 4096 ? 1 : (TM * TN == 4096 ? (k3M ? 3 : 4) : 2)))
 zgemm_grouped_kernel(const ZGroup g) {
   static_assert(!(k3M && kF32), "3M products are built for FP64 only");
-  using S = TileShape<TM, TN, kF32>;
+  using S = TileShape<TM, TN, kF32, kTKo>;
   using LT = typename std::conditional<kF32, float2, double2>::type;
   using AccT = typename std::conditional<kF32, float4_t, double4_t>::type;
-  using L = Smem<TM, TN, LT>;
+  using L = Smem<TM, TN, LT, kTKo>;
   constexpr int TK = S::kTK;
   constexpr int kSlab = S::kSlab;
   __shared__ __align__(16) char smem_raw[L::kBytes];
@@ -210,17 +215,17 @@ zgemm_grouped_kernel(const ZGroup g) {
     const ZTerm T = P.t[ti];
     const int nk = (T.k + TK - 1) / TK;
     double2 ra[S::kAPer], rb[S::kBPer];
-    load_a<TM, TN, kF32>(T, e, m, m0, 0, tid, ra);
-    load_b<TM, TN, kF32>(T, e, n, n0, 0, tid, rb);
-    store_a<TM, TN, kF32, LT>(T, tid, ra, As(0));
-    store_b<TM, TN, kF32, LT>(T, tid, rb, Bs(0));
+    load_a<TM, TN, kF32, kTKo>(T, e, m, m0, 0, tid, ra);
+    load_b<TM, TN, kF32, kTKo>(T, e, n, n0, 0, tid, rb);
+    store_a<TM, TN, kF32, kTKo, LT>(T, tid, ra, As(0));
+    store_b<TM, TN, kF32, kTKo, LT>(T, tid, rb, Bs(0));
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
       const int cur = kt & 1;
       const bool more = (kt + 1 < nk);
       if (more) {
-        load_a<TM, TN, kF32>(T, e, m, m0, (kt + 1) * TK, tid, ra);
-        load_b<TM, TN, kF32>(T, e, n, n0, (kt + 1) * TK, tid, rb);
+        load_a<TM, TN, kF32, kTKo>(T, e, m, m0, (kt + 1) * TK, tid, ra);
+        load_b<TM, TN, kF32, kTKo>(T, e, n, n0, (kt + 1) * TK, tid, rb);
       }
       auto as = As(cur);
       auto bs = Bs(cur);
@@ -311,8 +316,8 @@ zgemm_grouped_kernel(const ZGroup g) {
         }
       }
       if (more) {
-        store_a<TM, TN, kF32, LT>(T, tid, ra, As(cur ^ 1));
-        store_b<TM, TN, kF32, LT>(T, tid, rb, Bs(cur ^ 1));
+        store_a<TM, TN, kF32, kTKo, LT>(T, tid, ra, As(cur ^ 1));
+        store_b<TM, TN, kF32, kTKo, LT>(T, tid, rb, Bs(cur ^ 1));
       }
       __syncthreads();
     }
@@ -420,7 +425,7 @@ zgemm_grouped_kernel(const ZGroup g) {
   }
 }
 
-template <int TM, int TN, int kRole, bool kF32 = false, bool k3M = false>
+template <int TM, int TN, int kRole, bool kF32 = false, bool k3M = false, int kTKo = 0>
 hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   ZGroup g = g0;
   int per = 0;  // workgroups per energy (all problems)
@@ -435,8 +440,8 @@ hipError_t launch_cfg(const ZGroup& g0, hipStream_t stream) {
   const int64_t total = (int64_t)per * g.nE;
   if (total == 0) return hipSuccess;
   if (total > 0x7fffffff) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole, kF32, k3M>), dim3((unsigned)total),
-                     dim3(TileShape<TM, TN, kF32>::kThreads), 0, stream, g);
+  hipLaunchKernelGGL((zgemm_grouped_kernel<TM, TN, kRole, kF32, k3M, kTKo>), dim3((unsigned)total),
+                     dim3(TileShape<TM, TN, kF32, kTKo>::kThreads), 0, stream, g);
   return hipGetLastError();
 }
 
@@ -468,7 +473,7 @@ hipError_t launch_zgemm(const ZGroup& g, int cfg, hipStream_t stream) {
     case kTile128x64: return launch_cfg<128, 64, 0>(g, stream);
     case kTile64x64_3M: return launch_cfg<64, 64, 0, false, true>(g, stream);
     case kTile32x32_3M: return launch_cfg<32, 32, 0, false, true>(g, stream);
-    case kTile128x64_3M: return launch_cfg<128, 64, 0, false, true>(g, stream);
+    case kTile128x64_3M: return launch_cfg<128, 64, 0, false, true, 16>(g, stream);
     default: return launch_cfg<64, 64, 0>(g, stream);
   }
 }

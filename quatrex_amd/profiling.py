@@ -21,11 +21,12 @@ complex inverse (``rgf_inv_z``) of E diagonally dominant N x N matrices and
 checks the first result against numpy.
 
 ``python -m quatrex_amd.profiling sweep --n-blocks N --block-size B --energies E``
-runs the FP64 selected solve of the synthetic device (default tiles, no
-autotuner launches) and prints the sweep products' average launch time,
-flops and algorithmic bytes per launch from the library's HIP-event profile,
-so that a kernel trace and HBM counters of the same command can be set
-beside them.
+runs the FP64 selected solve of the synthetic device (default tiles or
+``--tile``, no autotuner launches; ``--persistent 0/1`` picks the path for
+blocks up to 32, ``--mixed`` the complex64-product sweep without its check)
+and prints energies/s and the sweep products' average launch time, flops and
+algorithmic bytes per launch from the library's HIP-event profile, so that a
+kernel trace and HBM counters of the same command can be set beside them.
 """
 
 from __future__ import annotations
@@ -33,6 +34,7 @@ from __future__ import annotations
 import argparse
 import csv
 import sys
+import time
 from collections import defaultdict
 from pathlib import Path
 
@@ -169,9 +171,11 @@ def dbstats(path: str) -> None:
         print(f"{name[:78]:78s} {n:6d} {avg / 1e3:10.1f} {t / 1e6:10.2f} {100 * t / tot:6.1f}")
 
 
-def time_sweep(n_blocks: int, b: int, energies: int, reps: int) -> None:
+def time_sweep(n_blocks: int, b: int, energies: int, reps: int, tile: str | None = None,
+               persistent: int | None = None, mixed: bool = False) -> None:
     import torch
 
+    from quatrex_amd import _lib
     from quatrex_amd import solve as S
     from quatrex_amd import workloads as W
     from quatrex_amd.layout import Geometry
@@ -188,21 +192,40 @@ def time_sweep(n_blocks: int, b: int, energies: int, reps: int) -> None:
     GL = S.packed_empty(g, energies, dev, parts=("d", "u"))
     GG = S.packed_empty(g, energies, dev, parts=("d", "u"))
     ctx = S.default_context(dev)
+    if tile:
+        ctx.set_option(_lib.RGF_OPT_TILE_CFG, _lib.TILE_CFG_NAMES.index(tile))
+    if persistent is not None:
+        ctx.set_option(_lib.RGF_OPT_PERSISTENT, int(persistent))
+    # mixed: the complex64-product sweep without the FP64 check (its products
+    # are the kernels profiled)
+    kw = {"precision": "mixed", "mixed": S.MixedOptions(refine=0, fallback_tol=0.0)} if mixed else {}
     ctx.profile(True)
+    t = 0.0
     for r in range(reps + 1):
         if r == 1:
             ctx.profile_read(reset=True)  # the first solve is a warm-up
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
         S.selected_solve(g, A, (sld, slu), (sgd, sgu), g_lesser=GL, g_greater=GG, g_retarded=GR,
-                         ctx=ctx)
+                         ctx=ctx, **kw)
+    torch.cuda.synchronize()
+    if reps > 0:
+        t = time.perf_counter() - t0
     p = ctx.profile_read(reset=True)
     ctx.status()
-    n = max(p["gemm_launches"], 1)
-    ms = p["gemm_ms"] / n
-    print(f"sweep {n_blocks}x{b} energies={energies}: {p['gemm_launches']} product launches, "
-          f"{ms * 1e3:.1f} us, {p['gemm_flops'] / n / 1e9:.4f} GFLOP and "
-          f"{p['gemm_bytes'] / n / 1e6:.2f} MB (algorithmic) per launch: "
-          f"{p['gemm_flops'] / p['gemm_ms'] / 1e9:.2f} TFLOP/s, "
-          f"{p['gemm_bytes'] / p['gemm_ms'] / 1e6:.1f} GB/s algorithmic")
+    line = f"sweep {n_blocks}x{b} energies={energies}"
+    if t > 0:
+        line += f": {reps * energies / t:.1f} energies/s"
+    n = p["gemm_launches"]
+    if n and p["gemm_ms"] > 0:
+        ms = p["gemm_ms"] / n
+        line += (f"; {n} product launches, {ms * 1e3:.1f} us, {p['gemm_flops'] / n / 1e9:.4f} GFLOP "
+                 f"and {p['gemm_bytes'] / n / 1e6:.2f} MB (algorithmic) per launch: "
+                 f"{p['gemm_flops'] / p['gemm_ms'] / 1e9:.2f} TFLOP/s, "
+                 f"{p['gemm_bytes'] / p['gemm_ms'] / 1e6:.1f} GB/s algorithmic")
+    else:
+        line += "; no product launches (on-chip sweep)"
+    print(line, flush=True)
 
 
 def main(argv=None) -> None:
@@ -233,6 +256,10 @@ def main(argv=None) -> None:
     w.add_argument("--block-size", type=int, default=32)
     w.add_argument("--energies", type=int, default=4096)
     w.add_argument("--reps", type=int, default=2)
+    w.add_argument("--tile", default=None, help="GEMM tile configuration name (default: 64x64)")
+    w.add_argument("--persistent", type=int, default=None,
+                   help="1/0: on-chip sweep for blocks <= 32 on/off (default: library default)")
+    w.add_argument("--mixed", action="store_true", help="complex64 products, no FP64 check")
     args = ap.parse_args(argv)
     if args.cmd == "copy":
         copy_calibration(args.gib, args.reps)
@@ -245,7 +272,8 @@ def main(argv=None) -> None:
     elif args.cmd == "inv":
         time_inverse(args.n, args.energies, args.reps)
     elif args.cmd == "sweep":
-        time_sweep(args.n_blocks, args.block_size, args.energies, args.reps)
+        time_sweep(args.n_blocks, args.block_size, args.energies, args.reps, args.tile,
+                   args.persistent, args.mixed)
     else:
         summarize(args.paths)
 

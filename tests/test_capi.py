@@ -83,3 +83,16 @@ def test_workspace_holds_every_block_inverse(sizes):
     for E in (1, 3):
         need = E * (11 * bmax * bmax * 16 + max(_inverse_aux_need(b) for b in sizes))
         assert workspace_size(g, E) >= need, (sizes, E)
+
+
+def test_autotune_energy_count_exceeds_the_cache():
+    """The autotuner times products over an operand stack larger than the
+    256 MB Infinity Cache, within [64, 1024] energies."""
+    from quatrex_amd.autotune import default_energies
+
+    for b in (16, 32, 64, 128, 256, 384, 512, 1024):
+        E = default_energies(b)
+        assert 64 <= E <= 1024
+        assert E == 1024 or E == 64 or b * b * E * 16 >= 256 << 20
+    assert default_energies(256) == 256 and default_energies(512) == 64
+    assert default_energies(128) == 1024
