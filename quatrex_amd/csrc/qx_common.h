@@ -191,6 +191,7 @@ struct SmallProblem {
 };
 struct SmallLaunch {
   int32_t nprob, total_jobs;
+  int32_t bmax, pad_;  // largest block over the problems (<= 16: half-depth products)
   int* counter;  // work-queue head, reset by the launcher
   int* info;     // singular-pivot status word
   char* scratch;

@@ -892,6 +892,8 @@ int run_small(Ctx* c, std::vector<SmallSpec> specs, char* scratch) {
   SmallLaunch L{};
   L.nprob = (int)probs.size();
   L.total_jobs = (int)total;
+  L.bmax = 0;
+  for (const SmallSpec& sp : specs) L.bmax = std::max(L.bmax, (int32_t)sp.g->bmax);
   L.counter = static_cast<int*>(d_counter);
   L.info = c->info_dev;
   L.scratch = per_wg ? scratch : nullptr;

@@ -19,18 +19,27 @@
 // T1 = K g^<_i - B U^H term (the same products as the reference, grouped
 // to fit the nine slots).
 //
-// Product tiles: wave w computes the 16 x 16 output tile (w / 2, w % 2) with
-// v_mfma_f64_16x16x4f64, complex as three real products (3M: ar br, ai bi,
-// (ar + ai)(br + bi)) into three accumulators.  All operand fragments of a
-// product are read from the LDS slots (row-major, 33-element row stride:
-// conflict-free column reads) before its first MFMA, so the MFMA chain does
-// not wait on LDS between k-steps.  The input blocks of a step are loaded
-// from HBM together (one latency per step, not one per block), and the
-// global addends of the W product are prefetched into registers.
-// Blocks smaller than 32 are zero-padded; tiles that lie outside a
-// product's valid rows / columns skip their MFMAs and the k loop stops at the
-// valid inner size.  Inverted blocks carry an identity pad, which no product
-// reads (k stops at the valid size and the other operand's pad is zero).
+// Products run in rounds: a round is a set of products that do not depend
+// on one another, wave w computes one 16 x 16 output tile of each of them
+// (tile (w - p) mod 4 of product p, so that products smaller than 16 x 16
+// spread over the waves), and the round ends with one barrier.  A round whose
+// outputs overwrite one of its own operands waits for every wave once more
+// before writing.  Outputs that are only stored (G^R_ij, G^R_ji, G_ii,
+// G^<>_ij and their conjugate transposes) go from registers to HBM without
+// an LDS slot.
+//
+// Tile products: v_mfma_f64_16x16x4f64, complex as three real products (3M:
+// ar br, ai bi, (ar + ai)(br + bi)) into three accumulators; the sign of a
+// term and the conjugations of op(A) / op(B) are the MFMA's negate-A
+// modifier and the sign of the fragment sums, so no fragment is scaled.  The
+// inner index always runs over the 32 padded rows: blocks smaller than 32 are
+// zero-padded and inverted blocks carry an identity pad, and no valid output
+// element sees a pad product (a pad column of one operand meets a zero pad row
+// of the other, or an identity pad meets zeros), so only the pad region of an
+// output can differ from zero (it is never stored).  With every block at most
+// 16 the inner loop runs over 16 (the KS = 4 kernel).
+#include <utility>
+
 #include "qx_common.h"
 
 namespace qx {
@@ -46,7 +55,7 @@ typedef double2 Slot[kB][kLd];
 struct SmallShared {
   Slot s[kNS];
   int piv[kB];     // pivot row of each elimination step
-  int step_of[kB]; // step at which each row was the pivot (-1: not yet)
+  int step_of[kB]; // step at which each row was the pivot
   double red[2][4];
   int job;
 };
@@ -55,8 +64,10 @@ struct Acc {
   double4_t p1, p2, p3;  // ar br, ai bi, (ar + ai)(br + bi)
 };
 
+template <bool NEG>
 __device__ inline double4_t mf(double a, double b, double4_t c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  // f64 MFMA with the BLGP field used as the negate-A modifier
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, NEG ? 1 : 0);
 }
 
 __device__ inline Acc acc_zero() {
@@ -67,33 +78,29 @@ __device__ inline Acc acc_zero() {
   return a;
 }
 
-// acc += sgn * op(A) op(B) over the inner index range [0, k) for this wave's
-// tile (tr, tc); op = plain or conjugate transpose (ah / bh).  Every
-// fragment is loaded first (k-steps beyond the valid range read in-bounds
-// pad and are zeroed by selection), then the MFMAs run back to back.
-template <bool AH, bool BH>
-__device__ inline void mma(Acc& c, const Slot& A, const Slot& B, int k, double sgn, int tr, int tc,
-                           int lane) {
+// acc += (NEG ? -1 : 1) op(A) op(B) over KS k-steps of 4 for this wave's tile
+// (tr, tc); op = plain or conjugate transpose (AH / BH).  With op(A) =
+// (ar, sa ai) and op(B) = (br, sb bi): P1 += ar br, P2 += sa sb ai bi,
+// P3 += (ar + sa ai)(br + sb bi), each negated for NEG.  All fragments are read
+// before the first MFMA, so the MFMA chain does not wait on LDS.
+template <int KS, bool AH, bool BH, bool NEG>
+__device__ inline void mma(Acc& c, const Slot& A, const Slot& B, int tr, int tc, int lane) {
   const int l16 = lane & 15, kq = lane >> 4;
   const int row = tr * 16 + l16, col = tc * 16 + l16;
-  const int nks = (k + 3) >> 2;
-  double ax[kB / 4], ay[kB / 4], bx[kB / 4], by[kB / 4];
+  double2 a[KS], b[KS];
 #pragma unroll
-  for (int s = 0; s < kB / 4; ++s) {
+  for (int s = 0; s < KS; ++s) {
     const int kk = s * 4 + kq;
-    const double2 a = AH ? A[kk][row] : A[row][kk];
-    const double2 b = BH ? B[col][kk] : B[kk][col];
-    const bool on = s < nks;
-    ax[s] = on ? sgn * a.x : 0.0;
-    ay[s] = on ? (AH ? -sgn * a.y : sgn * a.y) : 0.0;
-    bx[s] = on ? b.x : 0.0;
-    by[s] = on ? (BH ? -b.y : b.y) : 0.0;
+    a[s] = AH ? A[kk][row] : A[row][kk];
+    b[s] = BH ? B[col][kk] : B[kk][col];
   }
 #pragma unroll
-  for (int s = 0; s < kB / 4; ++s) {
-    c.p1 = mf(ax[s], bx[s], c.p1);
-    c.p2 = mf(ay[s], by[s], c.p2);
-    c.p3 = mf(ax[s] + ay[s], bx[s] + by[s], c.p3);
+  for (int s = 0; s < KS; ++s) {
+    const double sa = AH ? a[s].x - a[s].y : a[s].x + a[s].y;
+    const double sb = BH ? b[s].x - b[s].y : b[s].x + b[s].y;
+    c.p1 = mf<NEG>(a[s].x, b[s].x, c.p1);
+    c.p2 = mf<NEG != (AH != BH)>(a[s].y, b[s].y, c.p2);
+    c.p3 = mf<NEG>(sa, sb, c.p3);
   }
 }
 
@@ -103,6 +110,15 @@ __device__ inline void tile_vals(const Acc& c, double2 (&v)[4]) {
   for (int r = 0; r < 4; ++r) v[r] = cmk(c.p1[r] - c.p2[r], c.p3[r] - c.p1[r] - c.p2[r]);
 }
 
+__device__ inline int tile_row(int tr, int r, int lane) { return tr * 16 + 4 * r + (lane >> 4); }
+__device__ inline int tile_col(int tc, int lane) { return tc * 16 + (lane & 15); }
+// Tile of product p of a round computed by `wave`.
+__device__ inline void tile_of(int p, int wave, int& tr, int& tc) {
+  const int t = (wave - p) & 3;
+  tr = t >> 1;
+  tc = t & 1;
+}
+
 // Element of a global block (row-major, ld), zero outside rows x cols.
 __device__ inline double2 gld(const double2* F, int ld, int rows, int cols, int r, int c) {
   return (F && r < rows && c < cols) ? F[(int64_t)r * ld + c] : cmk(0.0, 0.0);
@@ -110,169 +126,291 @@ __device__ inline double2 gld(const double2* F, int ld, int rows, int cols, int 
 
 // One product's epilogue ingredients: up to three addends
 //   c1 * F[r][c] + c2 * conj(F[c][r])
-// from an LDS slot (s) or a global block (g, ld, rows, cols).
+// from an LDS slot (s), or this thread's four tile elements prefetched into
+// registers (r).
 struct Add {
   const Slot* s;
-  const double2* g;
-  int ld, rows, cols;
   double c1, c2;  // real coefficients suffice for every use here
-  const double2* r;  // this thread's four tile elements, prefetched into registers
+  const double2* r;
 };
-__device__ inline Add add_slot(const Slot& s, double c1, double c2 = 0.0) {
-  return Add{&s, nullptr, 0, 0, 0, c1, c2, nullptr};
-}
-__device__ inline Add add_glob(const double2* g, int ld, int rows, int cols, double c1) {
-  return Add{nullptr, g, ld, rows, cols, c1, 0.0, nullptr};
-}
-__device__ inline Add add_regs(const double2* r) { return Add{nullptr, nullptr, 0, 0, 0, 1.0, 0.0, r}; }
-__device__ inline Add add_none() { return Add{nullptr, nullptr, 0, 0, 0, 0.0, 0.0, nullptr}; }
+__device__ inline Add add_slot(const Slot& s, double c1, double c2 = 0.0) { return Add{&s, c1, c2, nullptr}; }
+__device__ inline Add add_regs(const double2* r) { return Add{nullptr, 1.0, 0.0, r}; }
+__device__ inline Add add_none() { return Add{nullptr, 0.0, 0.0, nullptr}; }
 
-// Element (row, col) of this wave's tile that register r of `lane` holds.
-__device__ inline int tile_row(int tr, int r, int lane) { return tr * 16 + 4 * r + (lane >> 4); }
-__device__ inline int tile_col(int tc, int lane) { return tc * 16 + (lane & 15); }
-
-__device__ inline double2 add_val(const Add& a, int r, int c) {
+__device__ inline double2 add_val(const Add& a, int q, int r, int c) {
+  if (a.r) return a.r[q];
   double2 out = cmk(0.0, 0.0);
   if (a.s) {
     const Slot& s = *a.s;
-    if (a.c1 != 0.0) out = cadd(out, cmk(a.c1 * s[r][c].x, a.c1 * s[r][c].y));
+    if (a.c1 != 0.0) out = cmk(a.c1 * s[r][c].x, a.c1 * s[r][c].y);
     if (a.c2 != 0.0) {
       const double2 t = s[c][r];
       out = cadd(out, cmk(a.c2 * t.x, -a.c2 * t.y));
     }
-  } else if (a.g) {
-    const double2 t = gld(a.g, a.ld, a.rows, a.cols, r, c);
-    out = cmk(a.c1 * t.x, a.c1 * t.y);
   }
   return out;
 }
 
+// Where a product's tile goes: an LDS slot (whole padded tile), a global
+// block (valid rows x cols, row-major, scaled) and / or the conjugate
+// transpose of the valid part into a second global block (gt[c][r]).
+struct Dst {
+  int slot = -1;
+  double2* g = nullptr;
+  int ld = 0, rows = 0, cols = 0;
+  double gs = 1.0;
+  double2* gt = nullptr;
+  int ldt = 0;
+  double ts = 1.0;
+  __device__ Dst glob(double2* p, int ld_, int rows_, int cols_, double s = 1.0) const {
+    Dst d = *this;
+    d.g = p;
+    d.ld = ld_;
+    d.rows = rows_;
+    d.cols = cols_;
+    d.gs = s;
+    return d;
+  }
+  __device__ Dst globT(double2* p, int ldt_, double s) const {
+    Dst d = *this;
+    d.gt = p;
+    d.ldt = ldt_;
+    d.ts = s;
+    return d;
+  }
+};
+__device__ inline Dst to_slot(int s) {
+  Dst d;
+  d.slot = s;
+  return d;
+}
+__device__ inline Dst to_glob(double2* p, int ld, int rows, int cols, double s = 1.0) {
+  return Dst{}.glob(p, ld, rows, cols, s);
+}
+
+// A product of a round: D (m x n valid) = terms + addends.  `partial`: the
+// value after the first `partial_after` addends also goes there (padded
+// 32 x 32 layout, global).  `on` = false skips the product (uniform).
+template <typename T>
+struct Prod {
+  bool on;
+  int m, n;
+  T terms;  // void(Acc&, int tr, int tc, int lane)
+  Dst d;
+  Add a0, a1, a2;
+  double2* partial;
+  int partial_after;
+};
+template <typename T>
+__device__ inline Prod<T> prod(int m, int n, T terms, Dst d, Add a0 = add_none(), Add a1 = add_none(),
+                               Add a2 = add_none(), double2* partial = nullptr, int partial_after = 0,
+                               bool on = true) {
+  return Prod<T>{on, m, n, terms, d, a0, a1, a2, partial, partial_after};
+}
+
+template <int I, typename T>
+__device__ inline void compute(const Prod<T>& p, double2 (&v)[4], int wave, int lane) {
+  int tr, tc;
+  tile_of(I, wave, tr, tc);
+  Acc acc = acc_zero();
+  if (p.on && tr * 16 < p.m && tc * 16 < p.n) p.terms(acc, tr, tc, lane);
+  tile_vals(acc, v);
+  if (!p.on) return;
+  const Add adds[3] = {p.a0, p.a1, p.a2};
+  const int col = tile_col(tc, lane);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (p.partial && a == p.partial_after) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) p.partial[tile_row(tr, r, lane) * kB + col] = v[r];
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[r] = cadd(v[r], add_val(adds[a], r, tile_row(tr, r, lane), col));
+  }
+}
+
+template <int I, typename T>
+__device__ inline void emit(SmallShared& sh, const Prod<T>& p, const double2 (&v)[4], int wave,
+                            int lane) {
+  if (!p.on) return;
+  int tr, tc;
+  tile_of(I, wave, tr, tc);
+  const int col = tile_col(tc, lane);
+  const Dst& d = p.d;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = tile_row(tr, r, lane);
+    if (d.slot >= 0) sh.s[d.slot][row][col] = v[r];
+    if (d.g && row < d.rows && col < d.cols)
+      d.g[(int64_t)row * d.ld + col] = cmk(d.gs * v[r].x, d.gs * v[r].y);
+    if (d.gt && row < d.rows && col < d.cols)
+      d.gt[(int64_t)col * d.ldt + row] = cmk(d.ts * v[r].x, -d.ts * v[r].y);
+  }
+}
+
+template <typename... P, size_t... I>
+__device__ inline void round_impl(SmallShared& sh, bool sync_first, std::index_sequence<I...>,
+                                  const P&... ps) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  double2 v[sizeof...(P)][4];
+  (compute<I>(ps, v[I], wave, lane), ...);
+  if (sync_first) __syncthreads();
+  (emit<I>(sh, ps, v[I], wave, lane), ...);
+  __syncthreads();
+}
+
+// Runs the products of one round and publishes their outputs (trailing
+// barrier).  `sync_first`: wait for every wave between computing and writing,
+// needed when an output slot is an operand or slot addend of any product of
+// the round (other than its own elements as an addend).
+template <typename... P>
+__device__ inline void run_round(SmallShared& sh, bool sync_first, const P&... ps) {
+  round_impl(sh, sync_first, std::index_sequence_for<P...>{}, ps...);
+}
+
 // Zero-filled loads of global blocks (rows x cols, ld) into slots, with
 // c2 * src2 (same shape; optional) added and `pad` on the diagonal beyond the
-// valid square (the identity pad of a block about to be inverted).  The
-// global loads of all NB blocks are issued before the barrier that protects
-// the slots' previous readers, so the blocks share one memory latency; the
-// caller synchronises before reading the slots.
+// valid square (the identity pad of a block about to be inverted).  Thread t
+// moves elements t + 256 q (q < 4) of the padded 32 x 32 block.
 struct Blk {
-  Slot* s;
+  int s;
   const double2* src;
   int ld, rows, cols;
   const double2* src2;
   double c2, pad;
 };
-__device__ inline Blk blk(Slot& s, const double2* src, int ld, int rows, int cols,
+__device__ inline Blk blk(int s, const double2* src, int ld, int rows, int cols,
                           const double2* src2 = nullptr, double c2 = 0.0, double pad = 0.0) {
-  return Blk{&s, src, ld, rows, cols, src2, c2, pad};
+  return Blk{s, src, ld, rows, cols, src2, c2, pad};
 }
-template <int NB>
-__device__ inline void load_blks(const Blk (&b)[NB]) {
+__device__ inline void fetch_blk(const Blk& b, double2 (&v)[4]) {
   const int tid = threadIdx.x;
-  double2 v[NB][kB * kB / kThreads];
 #pragma unroll
-  for (int i = 0; i < NB; ++i) {
-#pragma unroll
-    for (int q = 0; q < kB * kB / kThreads; ++q) {
-      const int idx = tid + q * kThreads;
-      const int r = idx >> 5, c = idx & 31;
-      double2 x = gld(b[i].src, b[i].ld, b[i].rows, b[i].cols, r, c);
-      if (b[i].src2 && r < b[i].rows && c < b[i].cols) {
-        const double2 w = b[i].src2[(int64_t)r * b[i].ld + c];
-        x = cmk(x.x + b[i].c2 * w.x, x.y + b[i].c2 * w.y);
-      }
-      if (r == c && r >= b[i].rows) x = cmk(b[i].pad, 0.0);
-      v[i][q] = x;
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * kThreads;
+    const int r = idx >> 5, c = idx & 31;
+    double2 x = gld(b.src, b.ld, b.rows, b.cols, r, c);
+    if (b.src2 && r < b.rows && c < b.cols) {
+      const double2 w = b.src2[(int64_t)r * b.ld + c];
+      x = cmk(x.x + b.c2 * w.x, x.y + b.c2 * w.y);
     }
+    if (r == c && r >= b.rows) x = cmk(b.pad, 0.0);
+    v[q] = x;
   }
+}
+__device__ inline void commit_blk(SmallShared& sh, int s, const double2 (&v)[4]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int idx = tid + q * kThreads;
+    sh.s[s][idx >> 5][idx & 31] = v[q];
+  }
+}
+// The global loads of all NB blocks are issued before the barrier that
+// protects the slots' previous readers, so the blocks share one memory
+// latency; the caller synchronises before reading the slots.
+template <int NB>
+__device__ inline void load_blks(SmallShared& sh, const Blk (&b)[NB]) {
+  double2 v[NB][4];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) fetch_blk(b[i], v[i]);
   __syncthreads();  // the slots' previous content may still be read
 #pragma unroll
-  for (int i = 0; i < NB; ++i) {
-#pragma unroll
-    for (int q = 0; q < kB * kB / kThreads; ++q) {
-      const int idx = tid + q * kThreads;
-      (*b[i].s)[idx >> 5][idx & 31] = v[i][q];
-    }
-  }
+  for (int i = 0; i < NB; ++i) commit_blk(sh, b[i].s, v[i]);
 }
-__device__ inline void load_blk(Slot& s, const double2* src, int ld, int rows, int cols,
-                                const double2* src2 = nullptr, double c2 = 0.0, double pad = 0.0) {
-  const Blk b[1] = {blk(s, src, ld, rows, cols, src2, c2, pad)};
-  load_blks<1>(b);
+__device__ inline void load_blk(SmallShared& sh, int s, const double2* src, int ld, int rows, int cols) {
+  const Blk b[1] = {blk(s, src, ld, rows, cols)};
+  load_blks<1>(sh, b);
 }
 
-// Global write of a slot's valid part.  mode 0: dst = scale * s;
-// 1: dst[r][c] = scale * conj(s[c][r]) (dst is cols x rows of the slot);
+// Global write of a slot's valid part after a barrier.  mode 0: dst = s;
 // 2: dst = (s - s^H) / 2 (square).
-// Mode 0 reads, per thread, exactly the elements the thread wrote as a
-// product epilogue (wave tile mapping), so a product may overwrite the slot
-// right after without a barrier; modes 1 and 2 read across threads.
-__device__ inline void store_blk(double2* dst, int ld, int rows, int cols, const Slot& s, int mode,
-                                 double scale = 1.0) {
+__device__ inline void store_blk(double2* dst, int ld, int rows, int cols, const Slot& s, int mode) {
   if (!dst) return;
   const int tid = threadIdx.x;
-  if (mode == 0) {
-    const int lane = tid & 63, wave = tid >> 6;
-    const int tr = wave >> 1, tc = wave & 1, c = tile_col(tc, lane);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = tile_row(tr, q, lane);
-      if (r < rows && c < cols) dst[(int64_t)r * ld + c] = cmk(scale * s[r][c].x, scale * s[r][c].y);
-    }
-    return;
-  }
 #pragma unroll
   for (int q = 0; q < kB * kB / kThreads; ++q) {
     const int idx = tid + q * kThreads;
     const int r = idx >> 5, c = idx & 31;
     if (r < rows && c < cols) {
-      double2 v;
-      if (mode == 0) {
-        v = cmk(scale * s[r][c].x, scale * s[r][c].y);
-      } else if (mode == 1) {
-        v = cmk(scale * s[c][r].x, -scale * s[c][r].y);
-      } else {
-        const double2 u = s[r][c], l = s[c][r];
-        v = cmk(0.5 * (u.x - l.x), 0.5 * (u.y + l.y));
+      double2 v = s[r][c];
+      if (mode == 2) {
+        const double2 l = s[c][r];
+        v = cmk(0.5 * (v.x - l.x), 0.5 * (v.y + l.y));
       }
       dst[(int64_t)r * ld + c] = v;
     }
   }
 }
 
-// In-place Gauss-Jordan inverse of the n x n valid part of a slot (identity
-// pad beyond n), partial pivoting without moving rows: step c eliminates
-// column c with the unused row of largest |re| + |im| (first on ties); the
-// result is permuted back at the end, Out[i][j] = M[pi(i)][pi^-1(j)].
-// Thread t owns row t / 8 and columns 4 (t % 8) .. 4 (t % 8) + 3.
-__device__ void invert(SmallShared& sh, Slot& M, int n, int* info, int tag, int64_t energy) {
+// 1/d for the pivot: conj(d) / |d|^2 with the reciprocal of |d|^2 from
+// v_rcp_f64 and two Newton steps; outside the range where |d|^2 neither
+// overflows nor loses precision the scaled division of crecip.  Every thread
+// passes the same d, so the branch is uniform.
+__device__ inline double2 pivot_recip(double2 d) {
+  const double m = d.x * d.x + d.y * d.y;
+  if (!(m > 1e-280 && m < 1e280)) return crecip(d);
+  double r = __builtin_amdgcn_rcp(m);
+  r = fma(r, fma(-m, r, 1.0), r);
+  r = fma(r, fma(-m, r, 1.0), r);
+  return cmk(d.x * r, -d.y * r);
+}
+
+// Pivot key of a candidate row: |re| + |im| rounded to float, with the low
+// six bits replaced by a marker bit (every candidate key is nonzero) and
+// 31 - row (ties go to the lower row).  Non-negative floats order like their
+// bit patterns; NaN ranks as infinity.
+__device__ inline unsigned pivot_key(double2 v, int row) {
+  const float k = (float)(fabs(v.x) + fabs(v.y));
+  unsigned u = __float_as_uint(k);
+  if (k != k) u = 0x7f800000u;
+  return (u & ~63u) | 32u | (unsigned)(31 - row);
+}
+
+// Maximum over each row of 16 lanes, left in the row's last lane (row_shr
+// 1, 2, 4, 8; a lane whose source lies before its row keeps its own value).
+__device__ inline unsigned row_max16(unsigned v) {
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x111, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x112, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x114, 0xf, 0xf, false));
+  v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x118, 0xf, 0xf, false));
+  return v;
+}
+
+// Gauss-Jordan inverse of the n x n valid part of slot ms (identity pad
+// beyond n) with partial pivoting and no row moves: step c eliminates column
+// c with the unused row of largest |re| + |im| (as a float; first row on
+// ties); the result is permuted back at the end, Out[i][j] = M[pi(i)][pi^-1(j)].
+// Steps alternate between slot ms and slot ts (every element is rewritten
+// each step), so one barrier per column separates a step's reads from the
+// next step's writes.  Every wave finds the pivot itself from lanes 0..31,
+// which also keep the used-row flags.  Thread t owns row t / 8 and columns
+// 4 (t % 8) .. 4 (t % 8) + 3.
+__device__ void invert(SmallShared& sh, int ms, int ts, int n, int* info, int tag, int64_t energy) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int r = tid >> 3, c0 = (tid & 7) * 4;
-  if (tid < kB) {
-    sh.step_of[tid] = -1;
-    sh.piv[tid] = tid;
-  }
-  __syncthreads();
+  bool used = false;  // lanes < 32: row `lane` has been a pivot
+  int cur = ms, nxt = ts;
 #pragma unroll 1
   for (int c = 0; c < n; ++c) {
-    // every wave finds the pivot itself (no broadcast needed): lanes < n
-    // carry the candidates of column c
-    double key = -1.0;
-    if (lane < n && sh.step_of[lane] < 0) {
-      const double2 v = M[lane][c];
-      key = fabs(v.x) + fabs(v.y);
-      key = (key != key) ? HUGE_VAL : key;
-    }
-    double wmax = key;
+    const Slot& S = sh.s[cur];
+    Slot& D = sh.s[nxt];
+    unsigned key = 0;
+    if (lane < n && !used) key = pivot_key(S[lane][c], lane);
+    const double2 a = S[r][c];
+    double2 own[4];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
-    const unsigned long long hit = __ballot(key == wmax);
-    int p = (int)__builtin_ctzll(hit);
+    for (int j = 0; j < 4; ++j) own[j] = S[r][c0 + j];
+    key = row_max16(key);
+    const unsigned mx = max((unsigned)__builtin_amdgcn_readlane((int)key, 15),
+                            (unsigned)__builtin_amdgcn_readlane((int)key, 31));
+    int p = 31 - (int)(mx & 31u);
     p = p < n ? p : n - 1;
-    const double2 pv = M[p][c];
-    const double2 a = M[r][c];
+    if (lane == p) used = true;
+    const double2 pv = S[p][c];
     double2 prow[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) prow[j] = M[p][c0 + j];
-    __syncthreads();  // every read of step c is done before any write
+    for (int j = 0; j < 4; ++j) prow[j] = S[p][c0 + j];
     if (tid == 0) {
       sh.step_of[p] = c;
       sh.piv[c] = p;
@@ -281,39 +419,37 @@ __device__ void invert(SmallShared& sh, Slot& M, int n, int* info, int tag, int6
         info[2] = c;
       }
     }
-    const double2 inv = crecip(pv);
-    if (r < n) {
-      if (r == p) {
+    const double2 inv = pivot_recip(pv);
+    if (r == p) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) M[r][c0 + j] = (c0 + j == c) ? inv : cmul(prow[j], inv);
-      } else {
-        const double2 f = cmul(a, inv);
+      for (int j = 0; j < 4; ++j) D[r][c0 + j] = (c0 + j == c) ? inv : cmul(prow[j], inv);
+    } else {
+      const double2 f = cmul(a, inv);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int cc = c0 + j;
-          if (cc == c) M[r][cc] = cmk(-f.x, -f.y);
-          else if (cc < n) M[r][cc] = csub(M[r][cc], cmul(f, prow[j]));
-        }
-      }
+      for (int j = 0; j < 4; ++j)
+        D[r][c0 + j] = (c0 + j == c) ? cmk(-f.x, -f.y) : csub(own[j], cmul(f, prow[j]));
     }
     __syncthreads();
+    const int t = cur;
+    cur = nxt;
+    nxt = t;
   }
-  // permute back: Out[i][j] = M[pi(i)][pi^-1(j)]
+  // permute back into slot ms: Out[i][j] = M[pi(i)][pi^-1(j)]
   double2 v[4];
   const int sr = r < n ? sh.piv[r] : r;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int cc = c0 + j;
     const int sc = cc < n ? sh.step_of[cc] : cc;
-    v[j] = M[sr][sc];
+    v[j] = sh.s[cur][sr][sc];
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 4; ++j) M[r][c0 + j] = v[j];
+  for (int j = 0; j < 4; ++j) sh.s[ms][r][c0 + j] = v[j];
   __syncthreads();
 }
 
-// sign * (tr(X1 Y1) - tr(X2 Y2)) over the valid b x b parts; thread 0 gets it.
+// tr(X1 Y1) - tr(X2 Y2) over the valid b x b parts; every thread gets it.
 __device__ double2 trace2(SmallShared& sh, const Slot& X1, const Slot& Y1, const Slot& X2,
                           const Slot& Y2, int b) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -356,54 +492,21 @@ struct Slots {
   __device__ void give(int s) { used &= ~(1u << s); }
 };
 
-}  // namespace
-
-namespace {
-
-// D = sum of products (the `terms` callback) + addends for the whole 32 x 32
-// slot `dst`; the tile of wave w is (w / 2, w % 2) and tiles outside the
-// valid m x n skip their MFMAs.  `partial`: the value after the first
-// `partial_after` addends also goes there (padded 32 x 32 layout).  Each
-// thread writes the elements of D it computed; `sync_first` waits for every
-// wave before those writes, which is needed when D is an operand, a
-// transposed addend, or a slot other threads read since the last barrier
-// (a transposing or projecting store).  The trailing barrier publishes D.
-template <typename F>
-__device__ inline void product(SmallShared& sh, int dst, int m, int n, F&& terms, Add a0, Add a1,
-                               Add a2, double2* partial = nullptr, int partial_after = 0,
-                               bool sync_first = false) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int tr = wave >> 1, tc = wave & 1;
-  Acc acc = acc_zero();
-  if (tr * 16 < m && tc * 16 < n) terms(acc, tr, tc, lane);
-  double2 v[4];
-  tile_vals(acc, v);
-  const Add adds[3] = {a0, a1, a2};
-  const int col = tile_col(tc, lane);
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (partial && a == partial_after) {
-      // the value before the remaining addends (Sigma~ of the current),
-      // padded 32 x 32 layout
-#pragma unroll
-      for (int r = 0; r < 4; ++r) partial[tile_row(tr, r, lane) * kB + col] = v[r];
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (adds[a].r) v[r] = cadd(v[r], adds[a].r[r]);
-      else v[r] = cadd(v[r], add_val(adds[a], tile_row(tr, r, lane), col));
-    }
+// Product terms: T<AH, BH, NEG>(A, B) is acc += (-1)^NEG op(A) op(B).
+#define QX_T(AH, BH, NEG, A_, B_)                                                   \
+  [&](Acc& ac, int tr, int tc, int lane) {                                          \
+    mma<KS, AH, BH, NEG>(ac, sh.s[A_], sh.s[B_], tr, tc, lane);                     \
   }
-  if (sync_first) __syncthreads();
-  Slot& D = sh.s[dst];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) D[tile_row(tr, r, lane)][col] = v[r];
-  __syncthreads();
-}
+#define QX_T2(AH1, BH1, NEG1, A1, B1, AH2, BH2, NEG2, A2, B2)                       \
+  [&](Acc& ac, int tr, int tc, int lane) {                                          \
+    mma<KS, AH1, BH1, NEG1>(ac, sh.s[A1], sh.s[B1], tr, tc, lane);                  \
+    mma<KS, AH2, BH2, NEG2>(ac, sh.s[A2], sh.s[B2], tr, tc, lane);                  \
+  }
 
+template <int KS>
 __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L) {
   __shared__ SmallShared sh;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double2* wg_scratch = L.scratch ? (double2*)(L.scratch + (int64_t)blockIdx.x * L.scratch_per_wg) : nullptr;
   for (;;) {
     if (tid == 0) sh.job = atomicAdd(L.counter, 1);
@@ -442,34 +545,27 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
     {
       const int b = bs[0];
       cg = S.take();
-      const int ws[2] = {S.take(), S.take()};
+      const int w0 = S.take(), w1 = S.take();
       {
-        const Blk blks[3] = {blk(sh.s[cg], D_(P.Ad, 0), b, b, b, OB(P.obr, 0), -1.0, 1.0),
-                             blk(sh.s[ws[0]], D_(P.SLd, 0), b, b, b, OB(P.obl, 0), 1.0),
-                             blk(sh.s[ws[1]], D_(P.SGd, 0), b, b, b, OB(P.obg, 0), 1.0)};
-        load_blks<3>(blks);
+        const Blk blks[3] = {blk(cg, D_(P.Ad, 0), b, b, b, OB(P.obr, 0), -1.0, 1.0),
+                             blk(w0, D_(P.SLd, 0), b, b, b, OB(P.obl, 0), 1.0),
+                             blk(w1, D_(P.SGd, 0), b, b, b, OB(P.obg, 0), 1.0)};
+        load_blks<3>(sh, blks);
       }
+      const int vl = S.take(), vg = S.take();
       __syncthreads();
-      invert(sh, sh.s[cg], b, L.info, 1, eg);
+      invert(sh, cg, vl, b, L.info, 1, eg);
       store_blk(Gd(0), Gld(0), b, b, sh.s[cg], 0);
-      double2* OUT[2] = {Dw(P.GLd, 0), Dw(P.GGd, 0)};
-      int res[2];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int w = ws[k];
-        const int v = S.take();
-        product(sh, v, b, b, [&](Acc& a, int tr, int tc, int lane) {
-          mma<false, false>(a, sh.s[cg], sh.s[w], b, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        product(sh, w, b, b, [&](Acc& a, int tr, int tc, int lane) {
-          mma<false, true>(a, sh.s[v], sh.s[cg], b, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        S.give(v);
-        store_blk(OUT[k], b, b, b, sh.s[w], 0);  // unprojected: the carried g^<>_0
-        res[k] = w;
-      }
-      cl = res[0];
-      cgg = res[1];
+      // V = g_0 W_0 ; g^<>_0 = V g_0^H (stored unprojected: the carried value)
+      run_round(sh, false, prod(b, b, QX_T(false, false, false, cg, w0), to_slot(vl)),
+                prod(b, b, QX_T(false, false, false, cg, w1), to_slot(vg)));
+      run_round(sh, false,
+                prod(b, b, QX_T(false, true, false, vl, cg), to_slot(w0).glob(Dw(P.GLd, 0), b, b, b)),
+                prod(b, b, QX_T(false, true, false, vg, cg), to_slot(w1).glob(Dw(P.GGd, 0), b, b, b)));
+      S.give(vl);
+      S.give(vg);
+      cl = w0;
+      cgg = w1;
     }
 
     // ---- forward sweep (rgf.py:241-302) ----
@@ -480,80 +576,69 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
       // Sigma^<_ij, Sigma^>_ij; the W addends Sigma^<>_jj + B^<>_j of this
       // thread's tile elements go to registers
       const int a = S.take(), t = S.take(), s = S.take(), sgl = S.take(), sgg = S.take();
-      {
-        const Blk b[5] = {blk(sh.s[a], U_(P.Al, i), bi, bj, bi),              // A_ji (bj x bi)
-                          blk(sh.s[t], U_(P.Au, i), bj, bi, bj),              // A_ij (bi x bj)
-                          blk(sh.s[s], D_(P.Ad, j), bj, bj, bj, OB(P.obr, j), -1.0, 1.0),
-                          blk(sh.s[sgl], U_(P.SLu, i), bj, bi, bj),           // Sigma^<_ij
-                          blk(sh.s[sgg], U_(P.SGu, i), bj, bi, bj)};          // Sigma^>_ij
-        load_blks<5>(b);
-      }
       double2 wadd[2][4];
       {
-        const int lane = tid & 63, wave = tid >> 6, c = tile_col(wave & 1, lane);
         const double2* SD[2] = {D_(P.SLd, j), D_(P.SGd, j)};
         const double2* BO[2] = {OB(P.obl, j), OB(P.obg, j)};
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
+        for (int k = 0; k < 2; ++k) {
+          int tr, tc;
+          tile_of(k, wave, tr, tc);
+          const int c = tile_col(tc, lane);
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int rr = tile_row(wave >> 1, r, lane);
+            const int rr = tile_row(tr, r, lane);
             wadd[k][r] = cadd(gld(SD[k], bj, bj, bj, rr, c), gld(BO[k], bj, bj, bj, rr, c));
           }
+        }
+        const Blk b[5] = {blk(a, U_(P.Al, i), bi, bj, bi),              // A_ji (bj x bi)
+                          blk(t, U_(P.Au, i), bj, bi, bj),              // A_ij (bi x bj)
+                          blk(s, D_(P.Ad, j), bj, bj, bj, OB(P.obr, j), -1.0, 1.0),
+                          blk(sgl, U_(P.SLu, i), bj, bi, bj),           // Sigma^<_ij
+                          blk(sgg, U_(P.SGu, i), bj, bi, bj)};          // Sigma^>_ij
+        load_blks<5>(sh, b);
       }
-      __syncthreads();
       const int pp = S.take();
-      product(sh, pp, bj, bi, [&](Acc& ac, int tr, int tc, int lane) {
-        mma<false, false>(ac, sh.s[a], sh.s[cg], bi, 1.0, tr, tc, lane);
-      }, add_none(), add_none(), add_none());
+      __syncthreads();
+      // P = A_ji g_i ; Y^<> = A_ji g^<>_i over the carried g^<>_i
+      run_round(sh, true, prod(bj, bi, QX_T(false, false, false, a, cg), to_slot(pp)),
+                prod(bj, bi, QX_T(false, false, false, a, cl), to_slot(cl)),
+                prod(bj, bi, QX_T(false, false, false, a, cgg), to_slot(cgg)));
       S.give(cg);
-      // S = A_jj - B^R_j - P A_ij (in place: each thread reads its own elements)
-      product(sh, s, bj, bj, [&](Acc& ac, int tr, int tc, int lane) {
-        mma<false, false>(ac, sh.s[pp], sh.s[t], bi, -1.0, tr, tc, lane);
-      }, add_slot(sh.s[s], 1.0), add_none(), add_none());
+      const int yl = cl, yg = cgg;
+      // S = A_jj - B^R_j - P A_ij (in place) ; Q^<> = P Sigma^<>_ij over Sigma_ij
+      run_round(sh, true, prod(bj, bj, QX_T(false, false, true, pp, t), to_slot(s), add_slot(sh.s[s], 1.0)),
+                prod(bj, bj, QX_T(false, false, false, pp, sgl), to_slot(sgl)),
+                prod(bj, bj, QX_T(false, false, false, pp, sgg), to_slot(sgg)));
       S.give(t);
-      invert(sh, sh.s[s], bj, L.info, j + 1, eg);
-      store_blk(Gd(j), Gld(j), bj, bj, sh.s[s], 0);
-      // lesser, then greater: Y = A_ji g^<>_i ; Q = P Sigma_ij ;
-      // W = Y A_ji^H + Q^H - Q [= Sigma~] + Sigma_jj + B_j ; g^<>_j = (g_j W) g_j^H
-      const int SU[2] = {sgl, sgg};
-      double2* OUT[2] = {Dw(P.GLd, j), Dw(P.GGd, j)};
-      int carried[2] = {cl, cgg};
-      int res[2];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int y = S.take();
-        product(sh, y, bj, bi, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[a], sh.s[carried[k]], bi, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        S.give(carried[k]);
-        const int q = S.take();
-        product(sh, q, bj, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[pp], sh.s[SU[k]], bi, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        S.give(SU[k]);
-        const int w = S.take();
-        product(sh, w, bj, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, true>(ac, sh.s[y], sh.s[a], bi, 1.0, tr, tc, lane);
-        }, add_slot(sh.s[q], -1.0, 1.0), add_regs(wadd[k]), add_none(), cur ? SigT(k, j) : nullptr, 1);
-        S.give(y);
-        S.give(q);
-        const int v = S.take();
-        product(sh, v, bj, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[s], sh.s[w], bj, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        product(sh, w, bj, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, true>(ac, sh.s[v], sh.s[s], bj, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        S.give(v);
-        store_blk(OUT[k], bj, bj, bj, sh.s[w], 0);  // unprojected left-connected g^<>_j
-        res[k] = w;
-      }
-      S.give(a);
+      const int ql = sgl, qg = sgg;
+      invert(sh, s, pp, bj, L.info, j + 1, eg);
       S.give(pp);
+      store_blk(Gd(j), Gld(j), bj, bj, sh.s[s], 0);
+      // W = Y A_ji^H + Q^H - Q [= Sigma~] + Sigma_jj + B_j
+      const int wl = S.take(), wg = S.take();
+      run_round(sh, false,
+                prod(bj, bj, QX_T(false, true, false, yl, a), to_slot(wl), add_slot(sh.s[ql], -1.0, 1.0),
+                     add_regs(wadd[0]), add_none(), cur ? SigT(0, j) : nullptr, 1),
+                prod(bj, bj, QX_T(false, true, false, yg, a), to_slot(wg), add_slot(sh.s[qg], -1.0, 1.0),
+                     add_regs(wadd[1]), add_none(), cur ? SigT(1, j) : nullptr, 1));
+      S.give(a);
+      S.give(yl);
+      S.give(yg);
+      S.give(ql);
+      S.give(qg);
+      // V = g_j W ; g^<>_j = V g_j^H (into the W slots, stored unprojected)
+      const int vl = S.take(), vg = S.take();
+      run_round(sh, false, prod(bj, bj, QX_T(false, false, false, s, wl), to_slot(vl)),
+                prod(bj, bj, QX_T(false, false, false, s, wg), to_slot(vg)));
+      run_round(sh, false,
+                prod(bj, bj, QX_T(false, true, false, vl, s), to_slot(wl).glob(Dw(P.GLd, j), bj, bj, bj)),
+                prod(bj, bj, QX_T(false, true, false, vg, s), to_slot(wg).glob(Dw(P.GGd, j), bj, bj, bj)));
+      S.give(vl);
+      S.give(vg);
       cg = s;
-      cl = res[0];
-      cgg = res[1];
+      cl = wl;
+      cgg = wg;
     }
 
     // ---- last block: G_{N-1} = g_{N-1}; stored diagonal G^<> projected ----
@@ -564,14 +649,14 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
       if (cur) {
         const int x = S.take(), y = S.take();
         if (N > 1) {
-          load_blk(sh.s[x], SigT(1, n1), kB, b, b);
-          load_blk(sh.s[y], SigT(0, n1), kB, b, b);
+          const Blk xy[2] = {blk(x, SigT(1, n1), kB, b, b), blk(y, SigT(0, n1), kB, b, b)};
+          load_blks<2>(sh, xy);
           __syncthreads();
           const double2 jv = trace2(sh, sh.s[x], sh.s[cl], sh.s[cgg], sh.s[y], b);
           if (tid == 0) P.cur[(int64_t)e * (N + 1) + n1] = jv;
         }
-        load_blk(sh.s[x], OB(P.obg, n1), b, b, b);
-        load_blk(sh.s[y], OB(P.obl, n1), b, b, b);
+        const Blk xy[2] = {blk(x, OB(P.obg, n1), b, b, b), blk(y, OB(P.obl, n1), b, b, b)};
+        load_blks<2>(sh, xy);
         __syncthreads();
         const double2 jn = trace2(sh, sh.s[x], sh.s[cl], sh.s[cgg], sh.s[y], b);
         if (tid == 0) P.cur[(int64_t)e * (N + 1) + N] = cmk(-jn.x, -jn.y);
@@ -591,124 +676,92 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
       const int j = i + 1;
       const int bi = bs[i], bj = bs[j];
       // the step's input blocks in one load: G_jj (retarded, full), g_i
-      // (left-connected), Sigma^<_ij, Sigma^>_ij, A_ij, A_ji
+      // (left-connected), Sigma^<_ij, Sigma^>_ij, A_ij, A_ji; the left-connected
+      // g^<>_i (unprojected) are fetched into registers one round before
+      // slots free up for them
       const int X = S.take(), g = S.take(), sgl = S.take(), sgg = S.take(), aij = S.take(),
                 aji = S.take();
+      const Blk gb[2] = {blk(0, Dw(P.GLd, i), bi, bi, bi), blk(0, Dw(P.GGd, i), bi, bi, bi)};
+      double2 gpre[4];
       {
-        const Blk b[6] = {blk(sh.s[X], Gd(j), Gld(j), bj, bj), blk(sh.s[g], Gd(i), Gld(i), bi, bi),
-                          blk(sh.s[sgl], U_(P.SLu, i), bj, bi, bj),
-                          blk(sh.s[sgg], U_(P.SGu, i), bj, bi, bj),
-                          blk(sh.s[aij], U_(P.Au, i), bj, bi, bj),
-                          blk(sh.s[aji], U_(P.Al, i), bi, bj, bi)};
-        load_blks<6>(b);
+        const Blk b[6] = {blk(X, Gd(j), Gld(j), bj, bj), blk(g, Gd(i), Gld(i), bi, bi),
+                          blk(sgl, U_(P.SLu, i), bj, bi, bj), blk(sgg, U_(P.SGu, i), bj, bi, bj),
+                          blk(aij, U_(P.Au, i), bj, bi, bj), blk(aji, U_(P.Al, i), bi, bj, bi)};
+        load_blks<6>(sh, b);
       }
       __syncthreads();
-      // B^<> = (g_i Sigma^<>_ij) G_jj^H
-      int Bk[2];
-      const int SU[2] = {sgl, sgg};
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int z = S.take();
-        product(sh, z, bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[g], sh.s[SU[k]], bi, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        S.give(SU[k]);
-        Bk[k] = S.take();
-        product(sh, Bk[k], bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, true>(ac, sh.s[z], sh.s[X], bj, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        S.give(z);
-      }
-      // U = g_i A_ij ; R = G_jj A_ji
-      const int u = S.take();
-      product(sh, u, bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
-        mma<false, false>(ac, sh.s[g], sh.s[aij], bi, 1.0, tr, tc, lane);
-      }, add_none(), add_none(), add_none());
-      S.give(aij);
-      const int rr = S.take();
-      product(sh, rr, bj, bi, [&](Acc& ac, int tr, int tc, int lane) {
-        mma<false, false>(ac, sh.s[X], sh.s[aji], bj, 1.0, tr, tc, lane);
-      }, add_none(), add_none(), add_none());
-      S.give(aji);
-      const int t = S.take();
-      // G^R_ij = -U G_jj (t is free now)
-      if (P.GRu) {
-        product(sh, t, bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[u], sh.s[X], bj, -1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        store_blk(Uw(P.GRu, i), bj, bi, bj, sh.s[t], 0);
-      }
-      S.give(X);
-      // G^R_ji = -R g_i
-      if (P.GRl) {
-        product(sh, t, bj, bi, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[rr], sh.s[g], bi, -1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        store_blk(Uw(P.GRl, i), bi, bj, bi, sh.s[t], 0);
-      }
-      // K = U R ; G_ii = g_i + K g_i
-      const int kk = S.take();
-      product(sh, kk, bi, bi, [&](Acc& ac, int tr, int tc, int lane) {
-        mma<false, false>(ac, sh.s[u], sh.s[rr], bj, 1.0, tr, tc, lane);
-      }, add_none(), add_none(), add_none());
-      product(sh, t, bi, bi, [&](Acc& ac, int tr, int tc, int lane) {
-        mma<false, false>(ac, sh.s[kk], sh.s[g], bi, 1.0, tr, tc, lane);
-      }, add_slot(sh.s[g], 1.0), add_none(), add_none());
-      store_blk(Gd(i), Gld(i), bi, bi, sh.s[t], 0);
-      S.give(t);
-      S.give(g);
-      // lesser, then greater
-      double2* OU[2] = {Uw(P.GLu, i), Uw(P.GGu, i)};
-      double2* OL[2] = {Uw(P.GLl, i), Uw(P.GGl, i)};
-      double2* OD[2] = {Dw(P.GLd, i), Dw(P.GGd, i)};
-      int Xc[2] = {Xl, Xg};
-      int res[2];
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        // T2 = U G^<>_jj
-        const int t2 = S.take();
-        product(sh, t2, bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[u], sh.s[Xc[k]], bj, 1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none());
-        S.give(Xc[k]);
-        const int gl = S.take();
-        load_blk(sh.s[gl], Dw(k == 0 ? P.GLd : P.GGd, i), bi, bi, bi);  // g^<>_i (unprojected)
-        __syncthreads();
-        // G^<>_ij = -g^<>_i R^H + B - T2 ; G^<>_ji = -(G^<>_ij)^H
-        const int st = S.take();
-        product(sh, st, bi, bj, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, true>(ac, sh.s[gl], sh.s[rr], bi, -1.0, tr, tc, lane);
-        }, add_slot(sh.s[Bk[k]], 1.0), add_slot(sh.s[t2], -1.0), add_none());
-        store_blk(OU[k], bj, bi, bj, sh.s[st], 0);
-        store_blk(OL[k], bi, bj, bi, sh.s[st], 1, -1.0);
-        // T1 = K g^<>_i - B U^H  (reuses st, which the transposing store read)
-        product(sh, st, bi, bi, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, false>(ac, sh.s[kk], sh.s[gl], bi, 1.0, tr, tc, lane);
-          mma<false, true>(ac, sh.s[Bk[k]], sh.s[u], bj, -1.0, tr, tc, lane);
-        }, add_none(), add_none(), add_none(), nullptr, 0, true);
-        S.give(Bk[k]);
-        // G^<>_ii = g^<>_i + T2 U^H + T1 - T1^H  (in place over g^<>_i)
-        product(sh, gl, bi, bi, [&](Acc& ac, int tr, int tc, int lane) {
-          mma<false, true>(ac, sh.s[t2], sh.s[u], bj, 1.0, tr, tc, lane);
-        }, add_slot(sh.s[gl], 1.0), add_slot(sh.s[st], 1.0, -1.0), add_none());
-        S.give(t2);
-        S.give(st);
-        store_blk(OD[k], bi, bi, bi, sh.s[gl], 2);
-        res[k] = gl;
-      }
+      // Z^<> = g_i Sigma^<>_ij ; U = g_i A_ij ; R = G_jj A_ji (over their B operands)
+      run_round(sh, true, prod(bi, bj, QX_T(false, false, false, g, sgl), to_slot(sgl)),
+                prod(bi, bj, QX_T(false, false, false, g, sgg), to_slot(sgg)),
+                prod(bi, bj, QX_T(false, false, false, g, aij), to_slot(aij)),
+                prod(bj, bi, QX_T(false, false, false, X, aji), to_slot(aji)));
+      const int zl = sgl, zg = sgg, u = aij, rr = aji;
+      fetch_blk(gb[0], gpre);
+      // B^<> = Z^<> G_jj^H ; G^R_ij = -U G_jj ; G^R_ji = -R g_i ; K = U R ;
+      // T2^<> = U G^<>_jj (over Z, G_jj and the carried G^<>_jj)
+      run_round(sh, true, prod(bi, bj, QX_T(false, true, false, zl, X), to_slot(zl)),
+                prod(bi, bj, QX_T(false, true, false, zg, X), to_slot(zg)),
+                prod(bi, bj, QX_T(false, false, true, u, X), to_glob(Uw(P.GRu, i), bj, bi, bj), add_none(),
+                     add_none(), add_none(), nullptr, 0, P.GRu != nullptr),
+                prod(bj, bi, QX_T(false, false, true, rr, g), to_glob(Uw(P.GRl, i), bi, bj, bi), add_none(),
+                     add_none(), add_none(), nullptr, 0, P.GRl != nullptr),
+                prod(bi, bi, QX_T(false, false, false, u, rr), to_slot(X)),
+                prod(bi, bj, QX_T(false, false, false, u, Xl), to_slot(Xl)),
+                prod(bi, bj, QX_T(false, false, false, u, Xg), to_slot(Xg)));
+      const int bl = zl, bg = zg, kk = X, t2l = Xl, t2g = Xg;
+      // g^<_i into the free slot
+      const int gl = S.take();
+      commit_blk(sh, gl, gpre);
+      fetch_blk(gb[1], gpre);
+      __syncthreads();
+      // G_ii = g_i + K g_i ; G^<_ij = -g^<_i R^H + B^< - T2^< (and G^<_ji =
+      // -(G^<_ij)^H) ; T1^< = K g^<_i - B^< U^H over g_i
+      run_round(sh, true,
+                prod(bi, bi, QX_T(false, false, false, kk, g), to_glob(Gd(i), Gld(i), bi, bi),
+                     add_slot(sh.s[g], 1.0)),
+                prod(bi, bj, QX_T(false, true, true, gl, rr),
+                     to_glob(Uw(P.GLu, i), bj, bi, bj).globT(Uw(P.GLl, i), bi, -1.0), add_slot(sh.s[bl], 1.0),
+                     add_slot(sh.s[t2l], -1.0)),
+                prod(bi, bi, QX_T2(false, false, false, kk, gl, false, true, true, bl, u), to_slot(g)));
+      const int t1l = g;
+      S.give(bl);
+      // g^>_i into the slot of B^<
+      const int gg = S.take();
+      commit_blk(sh, gg, gpre);
+      __syncthreads();
+      // G^>_ij, G^>_ji ; T1^> = K g^>_i - B^> U^H over K ;
+      // G^<_ii = g^<_i + T2^< U^H + T1^< - T1^<^H (in place)
+      run_round(sh, true,
+                prod(bi, bj, QX_T(false, true, true, gg, rr),
+                     to_glob(Uw(P.GGu, i), bj, bi, bj).globT(Uw(P.GGl, i), bi, -1.0), add_slot(sh.s[bg], 1.0),
+                     add_slot(sh.s[t2g], -1.0)),
+                prod(bi, bi, QX_T2(false, false, false, kk, gg, false, true, true, bg, u), to_slot(kk)),
+                prod(bi, bi, QX_T(false, true, false, t2l, u), to_slot(gl), add_slot(sh.s[gl], 1.0),
+                     add_slot(sh.s[t1l], 1.0, -1.0)));
+      const int t1g = kk;
+      // G^>_ii = g^>_i + T2^> U^H + T1^> - T1^>^H (in place)
+      run_round(sh, false,
+                prod(bi, bi, QX_T(false, true, false, t2g, u), to_slot(gg), add_slot(sh.s[gg], 1.0),
+                     add_slot(sh.s[t1g], 1.0, -1.0)));
+      store_blk(Dw(P.GLd, i), bi, bi, bi, sh.s[gl], 2);
+      store_blk(Dw(P.GGd, i), bi, bi, bi, sh.s[gg], 2);
+      S.give(t1g);
+      S.give(t1l);
+      S.give(bg);
       S.give(u);
       S.give(rr);
-      S.give(kk);
-      Xl = res[0];
-      Xg = res[1];
+      S.give(t2l);
+      S.give(t2g);
+      Xl = gl;
+      Xg = gg;
       if (cur) {
         const int x = S.take(), y = S.take();
         if (i >= 1) {
-          load_blk(sh.s[x], SigT(1, i), kB, bi, bi);
-          load_blk(sh.s[y], SigT(0, i), kB, bi, bi);
+          const Blk xy[2] = {blk(x, SigT(1, i), kB, bi, bi), blk(y, SigT(0, i), kB, bi, bi)};
+          load_blks<2>(sh, xy);
         } else {
-          load_blk(sh.s[x], OB(P.obg, 0), bi, bi, bi);
-          load_blk(sh.s[y], OB(P.obl, 0), bi, bi, bi);
+          const Blk xy[2] = {blk(x, OB(P.obg, 0), bi, bi, bi), blk(y, OB(P.obl, 0), bi, bi, bi)};
+          load_blks<2>(sh, xy);
         }
         __syncthreads();
         const double2 jv = trace2(sh, sh.s[x], sh.s[Xl], sh.s[Xg], sh.s[y], bi);
@@ -720,6 +773,9 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
     __syncthreads();
   }
 }
+
+#undef QX_T
+#undef QX_T2
 
 __global__ void reset_counter_kernel(int* c) {
   if (threadIdx.x == 0) c[0] = 0;
@@ -735,7 +791,10 @@ size_t sweep_small_scratch_per_wg(int n_blocks, bool need_g, bool current) {
 hipError_t launch_sweep_small(const SmallLaunch& L, int grid, hipStream_t stream) {
   if (L.total_jobs == 0) return hipSuccess;
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, L.counter);
-  hipLaunchKernelGGL(sweep_small_kernel, dim3(grid), dim3(kThreads), 0, stream, L);
+  if (L.bmax <= 16)
+    hipLaunchKernelGGL(sweep_small_kernel<4>, dim3(grid), dim3(kThreads), 0, stream, L);
+  else
+    hipLaunchKernelGGL(sweep_small_kernel<8>, dim3(grid), dim3(kThreads), 0, stream, L);
   return hipGetLastError();
 }
 
