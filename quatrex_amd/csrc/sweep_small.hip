@@ -54,8 +54,9 @@ typedef double2 Slot[kB][kLd];
 
 struct SmallShared {
   Slot s[kNS];
-  int piv[kB];     // pivot row of each elimination step
-  int step_of[kB]; // step at which each row was the pivot
+  double2 prow[2][16];  // pivot row of the current elimination step (halves)
+  int piv[kB];          // pivot row of each elimination step
+  int step_of[kB];      // step at which each row was the pivot
   double red[2][4];
   int job;
 };
@@ -119,9 +120,23 @@ __device__ inline void tile_of(int p, int wave, int& tr, int& tc) {
   tc = t & 1;
 }
 
+// Global-memory accesses through address-space-1 pointers: global (not
+// flat) instructions, which count on the vector-memory counter only, so an
+// LDS wait does not also wait for prefetches still in flight.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ inline double2 gget(const double2* p, int64_t i) {
+  const gdouble* q = (const gdouble*)p + 2 * i;
+  return cmk(q[0], q[1]);
+}
+__device__ inline void gput(double2* p, int64_t i, double2 v) {
+  gdouble* q = (gdouble*)p + 2 * i;
+  q[0] = v.x;
+  q[1] = v.y;
+}
+
 // Element of a global block (row-major, ld), zero outside rows x cols.
 __device__ inline double2 gld(const double2* F, int ld, int rows, int cols, int r, int c) {
-  return (F && r < rows && c < cols) ? F[(int64_t)r * ld + c] : cmk(0.0, 0.0);
+  return (F && r < rows && c < cols) ? gget(F, (int64_t)r * ld + c) : cmk(0.0, 0.0);
 }
 
 // One product's epilogue ingredients: up to three addends
@@ -222,7 +237,7 @@ __device__ inline void compute(const Prod<T>& p, double2 (&v)[4], int wave, int 
   for (int a = 0; a < 3; ++a) {
     if (p.partial && a == p.partial_after) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) p.partial[tile_row(tr, r, lane) * kB + col] = v[r];
+      for (int r = 0; r < 4; ++r) gput(p.partial, tile_row(tr, r, lane) * kB + col, v[r]);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) v[r] = cadd(v[r], add_val(adds[a], r, tile_row(tr, r, lane), col));
@@ -242,9 +257,9 @@ __device__ inline void emit(SmallShared& sh, const Prod<T>& p, const double2 (&v
     const int row = tile_row(tr, r, lane);
     if (d.slot >= 0) sh.s[d.slot][row][col] = v[r];
     if (d.g && row < d.rows && col < d.cols)
-      d.g[(int64_t)row * d.ld + col] = cmk(d.gs * v[r].x, d.gs * v[r].y);
+      gput(d.g, (int64_t)row * d.ld + col, cmk(d.gs * v[r].x, d.gs * v[r].y));
     if (d.gt && row < d.rows && col < d.cols)
-      d.gt[(int64_t)col * d.ldt + row] = cmk(d.ts * v[r].x, -d.ts * v[r].y);
+      gput(d.gt, (int64_t)col * d.ldt + row, cmk(d.ts * v[r].x, -d.ts * v[r].y));
   }
 }
 
@@ -291,7 +306,7 @@ __device__ inline void fetch_blk(const Blk& b, double2 (&v)[4]) {
     const int r = idx >> 5, c = idx & 31;
     double2 x = gld(b.src, b.ld, b.rows, b.cols, r, c);
     if (b.src2 && r < b.rows && c < b.cols) {
-      const double2 w = b.src2[(int64_t)r * b.ld + c];
+      const double2 w = gget(b.src2, (int64_t)r * b.ld + c);
       x = cmk(x.x + b.c2 * w.x, x.y + b.c2 * w.y);
     }
     if (r == c && r >= b.rows) x = cmk(b.pad, 0.0);
@@ -338,7 +353,7 @@ __device__ inline void store_blk(double2* dst, int ld, int rows, int cols, const
         const double2 l = s[c][r];
         v = cmk(0.5 * (v.x - l.x), 0.5 * (v.y + l.y));
       }
-      dst[(int64_t)r * ld + c] = v;
+      gput(dst, (int64_t)r * ld + c, v);
     }
   }
 }
@@ -347,24 +362,23 @@ __device__ inline void store_blk(double2* dst, int ld, int rows, int cols, const
 // v_rcp_f64 and two Newton steps; outside the range where |d|^2 neither
 // overflows nor loses precision the scaled division of crecip.  Every thread
 // passes the same d, so the branch is uniform.
+__device__ __noinline__ double2 crecip_slow(double2 d) { return crecip(d); }
 __device__ inline double2 pivot_recip(double2 d) {
   const double m = d.x * d.x + d.y * d.y;
-  if (!(m > 1e-280 && m < 1e280)) return crecip(d);
+  if (!(m > 1e-280 && m < 1e280)) return crecip_slow(d);
   double r = __builtin_amdgcn_rcp(m);
   r = fma(r, fma(-m, r, 1.0), r);
   r = fma(r, fma(-m, r, 1.0), r);
   return cmk(d.x * r, -d.y * r);
 }
 
-// Pivot key of a candidate row: |re| + |im| rounded to float, with the low
-// six bits replaced by a marker bit (every candidate key is nonzero) and
-// 31 - row (ties go to the lower row).  Non-negative floats order like their
-// bit patterns; NaN ranks as infinity.
+// Pivot key of a candidate row: the high word of |re| + |im| (non-negative
+// doubles order like their bit patterns; NaN above infinity), with its low six
+// bits replaced by a marker bit (every candidate key is nonzero) and 31 - row
+// (ties go to the lower row).
 __device__ inline unsigned pivot_key(double2 v, int row) {
-  const float k = (float)(fabs(v.x) + fabs(v.y));
-  unsigned u = __float_as_uint(k);
-  if (k != k) u = 0x7f800000u;
-  return (u & ~63u) | 32u | (unsigned)(31 - row);
+  const unsigned hi = (unsigned)(__double_as_longlong(fabs(v.x) + fabs(v.y)) >> 32);
+  return (hi & ~63u) | 32u | (unsigned)(31 - row);
 }
 
 // Maximum over each row of 16 lanes, left in the row's last lane (row_shr
@@ -377,76 +391,104 @@ __device__ inline unsigned row_max16(unsigned v) {
   return v;
 }
 
+// Value of lane `src` (uniform) of this wave.
+__device__ inline double readlane_d(double x, int src) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Gauss-Jordan inverse of the n x n valid part of slot ms (identity pad
-// beyond n) with partial pivoting and no row moves: step c eliminates column
-// c with the unused row of largest |re| + |im| (as a float; first row on
-// ties); the result is permuted back at the end, Out[i][j] = M[pi(i)][pi^-1(j)].
-// Steps alternate between slot ms and slot ts (every element is rewritten
-// each step), so one barrier per column separates a step's reads from the
-// next step's writes.  Every wave finds the pivot itself from lanes 0..31,
-// which also keep the used-row flags.  Thread t owns row t / 8 and columns
-// 4 (t % 8) .. 4 (t % 8) + 3.
-__device__ void invert(SmallShared& sh, int ms, int ts, int n, int* info, int tag, int64_t energy) {
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int r = tid >> 3, c0 = (tid & 7) * 4;
-  bool used = false;  // lanes < 32: row `lane` has been a pivot
-  int cur = ms, nxt = ts;
+// beyond n) with partial pivoting and no row moves, by ONE wave with the
+// matrix in registers (no barrier, so the other waves can work meanwhile):
+// step c eliminates column c with the unused row of largest |re| + |im|
+// (compared on the high word; first row on ties), and the result is permuted
+// back at the end, Out[i][j] = M[pi(i)][pi^-1(j)].  Lane l holds row l % 32,
+// columns 16 (l / 32) .. 16 (l / 32) + 15, rotated by one register position
+// per step (for free: the update writes register k + 1's new value into
+// register k), so that column c is always register 0 of the lanes of its
+// half and the step loop stays rolled with constant register indices.
+// Step c: those lanes search the column (DPP row maxima of the half's two
+// 16-lane rows), every lane reads the pivot value from the pivot lane, the
+// other half receives its row's column-c value by a lane swap, the pivot
+// row's two lanes publish it in LDS (the wave's LDS accesses complete in
+// order) and every lane applies
+//   row <- keep row - g prow,  keep = 1, g = m_rc / pivot  (keep = 0,
+//   g = -1/pivot for the pivot row),
+// then sets column c to -g.  Slot ts is scratch for the permutation back
+// into slot ms.
+__device__ __forceinline__ void invert_wave(SmallShared& sh, int ms, int ts, int n, int* info, int tag,
+                                            int64_t energy) {
+  const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+  double2 m[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) m[k] = sh.s[ms][r][16 * h + k];
+  bool used = false;
 #pragma unroll 1
   for (int c = 0; c < n; ++c) {
-    const Slot& S = sh.s[cur];
-    Slot& D = sh.s[nxt];
-    unsigned key = 0;
-    if (lane < n && !used) key = pivot_key(S[lane][c], lane);
-    const double2 a = S[r][c];
-    double2 own[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) own[j] = S[r][c0 + j];
+    const int hc = c >> 4;
+    // register 0: column c in half hc, column c + 16 - 32 hc in the other
+    unsigned key = (h == hc && r < n && !used) ? pivot_key(m[0], r) : 0u;
     key = row_max16(key);
-    const unsigned mx = max((unsigned)__builtin_amdgcn_readlane((int)key, 15),
-                            (unsigned)__builtin_amdgcn_readlane((int)key, 31));
+    const unsigned mx = max((unsigned)__builtin_amdgcn_readlane((int)key, 32 * hc + 15),
+                            (unsigned)__builtin_amdgcn_readlane((int)key, 32 * hc + 31));
     int p = 31 - (int)(mx & 31u);
     p = p < n ? p : n - 1;
-    if (lane == p) used = true;
-    const double2 pv = S[p][c];
-    double2 prow[4];
+    const bool me = r == p;
+    if (me) used = true;
+    // publish the pivot row (its two lanes) and fetch the partner half's
+    // column-c value before the reciprocal's dependent chain
+    if (me) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) prow[j] = S[p][c0 + j];
-    if (tid == 0) {
-      sh.step_of[p] = c;
+      for (int k = 0; k < 16; ++k) sh.prow[h][k] = m[k];
+    }
+    const double ox = __shfl_xor(m[0].x, 32), oy = __shfl_xor(m[0].y, 32);
+    const double2 pv = cmk(readlane_d(m[0].x, 32 * hc + p), readlane_d(m[0].y, 32 * hc + p));
+    if (lane == 0) {
       sh.piv[c] = p;
+      sh.step_of[p] = c;
       if (pv.x == 0.0 && pv.y == 0.0 && atomicCAS(info, 0, tag) == 0) {
         info[1] = (int)energy;
         info[2] = c;
       }
     }
     const double2 inv = pivot_recip(pv);
-    if (r == p) {
+    const double2 a = (h == hc) ? m[0] : cmk(ox, oy);  // this row's column-c value
+    const double2 f = cmul(a, inv);
+    const double keep = me ? 0.0 : 1.0;
+    const double2 g = me ? cmk(-inv.x, -inv.y) : f;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    asm volatile("" ::: "memory");
+    double2 nw[16];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) D[r][c0 + j] = (c0 + j == c) ? inv : cmul(prow[j], inv);
-    } else {
-      const double2 f = cmul(a, inv);
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        D[r][c0 + j] = (c0 + j == c) ? cmk(-f.x, -f.y) : csub(own[j], cmul(f, prow[j]));
+    for (int k = 0; k < 16; ++k) {
+      const double2 pr = sh.prow[h][k];
+      nw[k] = cmk(keep * m[k].x - (g.x * pr.x - g.y * pr.y), keep * m[k].y - (g.x * pr.y + g.y * pr.x));
     }
-    __syncthreads();
-    const int t = cur;
-    cur = nxt;
-    nxt = t;
+    if (h == hc) nw[0] = cmk(-g.x, -g.y);
+    // rotate by one position
+#pragma unroll
+    for (int k = 0; k < 15; ++k) m[k] = nw[k + 1];
+    m[15] = nw[0];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    asm volatile("" ::: "memory");
   }
-  // permute back into slot ms: Out[i][j] = M[pi(i)][pi^-1(j)]
-  double2 v[4];
+  // register k now holds column 16 h + ((k + n) mod 16); permute back
+  // through slot ts: Out[i][j] = M[pi(i)][pi^-1(j)]
+#pragma unroll
+  for (int k = 0; k < 16; ++k) sh.s[ts][r][16 * h + ((k + n) & 15)] = m[k];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  asm volatile("" ::: "memory");
   const int sr = r < n ? sh.piv[r] : r;
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int cc = c0 + j;
+  for (int k = 0; k < 16; ++k) {
+    const int cc = 16 * h + k;
     const int sc = cc < n ? sh.step_of[cc] : cc;
-    v[j] = sh.s[cur][sr][sc];
+    m[k] = sh.s[ts][sr][sc];
   }
-  __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 4; ++j) sh.s[ms][r][c0 + j] = v[j];
-  __syncthreads();
+  for (int k = 0; k < 16; ++k) sh.s[ms][r][16 * h + k] = m[k];
 }
 
 // tr(X1 Y1) - tr(X2 Y2) over the valid b x b parts; every thread gets it.
@@ -479,6 +521,30 @@ __device__ double2 trace2(SmallShared& sh, const Slot& X1, const Slot& Y1, const
                     sh.red[1][0] + sh.red[1][1] + sh.red[1][2] + sh.red[1][3]);
   __syncthreads();
   return out;
+}
+
+// W = Y A^H + Q^H - Q [= Sigma~, to `partial`] + addends (this lane's
+// elements of all four tiles, in registers), all four tiles by one wave.
+template <int KS>
+__device__ inline void w_product(SmallShared& sh, int y, int a, int q, int out, int m,
+                                 const double2 (&add)[4][4], double2* partial, int lane) {
+  const Add qa = add_slot(sh.s[q], -1.0, 1.0);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int tr = t >> 1, tc = t & 1;
+    Acc acc = acc_zero();
+    if (tr * 16 < m && tc * 16 < m) mma<KS, false, true, false>(acc, sh.s[y], sh.s[a], tr, tc, lane);
+    double2 v[4];
+    tile_vals(acc, v);
+    const int col = tile_col(tc, lane);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = tile_row(tr, r, lane);
+      v[r] = cadd(v[r], add_val(qa, r, row, col));
+      if (partial) gput(partial, row * kB + col, v[r]);
+      sh.s[out][row][col] = cadd(v[r], add[t][r]);
+    }
+  }
 }
 
 // Uniform slot allocator (every thread holds the same mask).
@@ -515,21 +581,28 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
     __syncthreads();
     if (job >= L.total_jobs) break;
     int p = 0;
-    while (p + 1 < L.nprob && job >= L.probs[p + 1].job0) ++p;
-    const SmallProblem& P = L.probs[p];
+    // the problem table and its index tables are read through the scalar
+    // (constant) path
+    typedef __attribute__((address_space(4))) const SmallProblem cprob;
+    const cprob* probs = (const cprob*)L.probs;
+    while (p + 1 < L.nprob && job >= probs[p + 1].job0) ++p;
+    const cprob& P = probs[p];
     const int e = job - P.job0;
     const int64_t eg = P.e_base + e;
     const int N = P.N;
-    const int* bs = P.bs;
-    const int64_t* od = P.od;
-    const int64_t* ou = P.ou;
+    const __attribute__((address_space(4))) int* bs = (const __attribute__((address_space(4))) int*)P.bs;
+    const __attribute__((address_space(4))) int64_t* od = (const __attribute__((address_space(4))) int64_t*)P.od;
+    const __attribute__((address_space(4))) int64_t* ou = (const __attribute__((address_space(4))) int64_t*)P.ou;
     auto D_ = [&](const double2* base, int i) { return base ? base + (int64_t)e * P.Dsz + od[i] : nullptr; };
     auto U_ = [&](const double2* base, int i) { return base ? base + (int64_t)e * P.Usz + ou[i] : nullptr; };
     auto Dw = [&](double2* base, int i) { return base ? base + (int64_t)e * P.Dsz + od[i] : nullptr; };
     auto Uw = [&](double2* base, int i) { return base ? base + (int64_t)e * P.Usz + ou[i] : nullptr; };
     auto OB = [&](const double2* const* arr, int i) -> const double2* {
-      if (!arr || !arr[i]) return nullptr;
-      return arr[i] + (int64_t)e * bs[i] * bs[i];
+      if (!arr) return nullptr;
+      typedef const double2* cptr;
+      typedef __attribute__((address_space(4))) const cptr ccptr;
+      const double2* b = ((const ccptr*)arr)[i];
+      return b ? b + (int64_t)e * bs[i] * bs[i] : nullptr;
     };
     // left-connected g_i: the G^R diagonal output, or this workgroup's scratch
     double2* gstore = P.GRd ? nullptr : wg_scratch;
@@ -554,7 +627,8 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
       }
       const int vl = S.take(), vg = S.take();
       __syncthreads();
-      invert(sh, cg, vl, b, L.info, 1, eg);
+      if (wave == 0) invert_wave(sh, cg, vl, b, L.info, 1, eg);
+      __syncthreads();
       store_blk(Gd(0), Gld(0), b, b, sh.s[cg], 0);
       // V = g_0 W_0 ; g^<>_0 = V g_0^H (stored unprojected: the carried value)
       run_round(sh, false, prod(b, b, QX_T(false, false, false, cg, w0), to_slot(vl)),
@@ -573,25 +647,24 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
       const int i = j - 1;
       const int bi = bs[i], bj = bs[j];
       // the step's input blocks in one load: A_ji, A_ij, A_jj - B^R_j,
-      // Sigma^<_ij, Sigma^>_ij; the W addends Sigma^<>_jj + B^<>_j of this
-      // thread's tile elements go to registers
+      // Sigma^<_ij, Sigma^>_ij; the W addends Sigma^<_jj + B^<_j (wave 0) and
+      // Sigma^>_jj + B^>_j (wave 1) of the lane's elements go to registers
       const int a = S.take(), t = S.take(), s = S.take(), sgl = S.take(), sgg = S.take();
-      double2 wadd[2][4];
+      double2 wadd[4][4];
       {
-        const double2* SD[2] = {D_(P.SLd, j), D_(P.SGd, j)};
-        const double2* BO[2] = {OB(P.obl, j), OB(P.obg, j)};
+        const double2* SD = wave == 0 ? D_(P.SLd, j) : D_(P.SGd, j);
+        const double2* BO = wave == 0 ? OB(P.obl, j) : OB(P.obg, j);
+        if (wave >= 2) SD = BO = nullptr;
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          int tr, tc;
-          tile_of(k, wave, tr, tc);
-          const int c = tile_col(tc, lane);
+        for (int tt = 0; tt < 4; ++tt) {
+          const int c = tile_col(tt & 1, lane);
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int rr = tile_row(tr, r, lane);
-            wadd[k][r] = cadd(gld(SD[k], bj, bj, bj, rr, c), gld(BO[k], bj, bj, bj, rr, c));
+            const int rr = tile_row(tt >> 1, r, lane);
+            wadd[tt][r] = cadd(gld(SD, bj, bj, bj, rr, c), gld(BO, bj, bj, bj, rr, c));
           }
         }
-        const Blk b[5] = {blk(a, U_(P.Al, i), bi, bj, bi),              // A_ji (bj x bi)
+      const Blk b[5] = {blk(a, U_(P.Al, i), bi, bj, bi),              // A_ji (bj x bi)
                           blk(t, U_(P.Au, i), bj, bi, bj),              // A_ij (bi x bj)
                           blk(s, D_(P.Ad, j), bj, bj, bj, OB(P.obr, j), -1.0, 1.0),
                           blk(sgl, U_(P.SLu, i), bj, bi, bj),           // Sigma^<_ij
@@ -612,16 +685,18 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
                 prod(bj, bj, QX_T(false, false, false, pp, sgg), to_slot(sgg)));
       S.give(t);
       const int ql = sgl, qg = sgg;
-      invert(sh, s, pp, bj, L.info, j + 1, eg);
+      // g_j = S^-1 by wave 3 while waves 0 and 1 form
+      // W^<> = Y^<> A_ji^H + Q^<>^H - Q^<> [= Sigma~] + Sigma^<>_jj + B^<>_j
+      const int wl = S.take(), wg = S.take();
+      if (wave == 3)
+        invert_wave(sh, s, pp, bj, L.info, j + 1, eg);
+      else if (wave == 0)
+        w_product<KS>(sh, yl, a, ql, wl, bj, wadd, cur ? SigT(0, j) : nullptr, lane);
+      else if (wave == 1)
+        w_product<KS>(sh, yg, a, qg, wg, bj, wadd, cur ? SigT(1, j) : nullptr, lane);
+      __syncthreads();
       S.give(pp);
       store_blk(Gd(j), Gld(j), bj, bj, sh.s[s], 0);
-      // W = Y A_ji^H + Q^H - Q [= Sigma~] + Sigma_jj + B_j
-      const int wl = S.take(), wg = S.take();
-      run_round(sh, false,
-                prod(bj, bj, QX_T(false, true, false, yl, a), to_slot(wl), add_slot(sh.s[ql], -1.0, 1.0),
-                     add_regs(wadd[0]), add_none(), cur ? SigT(0, j) : nullptr, 1),
-                prod(bj, bj, QX_T(false, true, false, yg, a), to_slot(wg), add_slot(sh.s[qg], -1.0, 1.0),
-                     add_regs(wadd[1]), add_none(), cur ? SigT(1, j) : nullptr, 1));
       S.give(a);
       S.give(yl);
       S.give(yg);
@@ -653,16 +728,16 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
           load_blks<2>(sh, xy);
           __syncthreads();
           const double2 jv = trace2(sh, sh.s[x], sh.s[cl], sh.s[cgg], sh.s[y], b);
-          if (tid == 0) P.cur[(int64_t)e * (N + 1) + n1] = jv;
+          if (tid == 0) gput(P.cur, (int64_t)e * (N + 1) + n1, jv);
         }
         const Blk xy[2] = {blk(x, OB(P.obg, n1), b, b, b), blk(y, OB(P.obl, n1), b, b, b)};
         load_blks<2>(sh, xy);
         __syncthreads();
         const double2 jn = trace2(sh, sh.s[x], sh.s[cl], sh.s[cgg], sh.s[y], b);
-        if (tid == 0) P.cur[(int64_t)e * (N + 1) + N] = cmk(-jn.x, -jn.y);
+        if (tid == 0) gput(P.cur, (int64_t)e * (N + 1) + N, cmk(-jn.x, -jn.y));
         if (N == 1) {
           // J_0 with the same (first = last) block
-          if (tid == 0) P.cur[(int64_t)e * (N + 1)] = jn;
+          if (tid == 0) gput(P.cur, (int64_t)e * (N + 1), jn);
         }
         S.give(x);
         S.give(y);
@@ -765,7 +840,7 @@ __global__ void __launch_bounds__(kThreads, 1) sweep_small_kernel(SmallLaunch L)
         }
         __syncthreads();
         const double2 jv = trace2(sh, sh.s[x], sh.s[Xl], sh.s[Xg], sh.s[y], bi);
-        if (tid == 0) P.cur[(int64_t)e * (N + 1) + i] = jv;
+        if (tid == 0) gput(P.cur, (int64_t)e * (N + 1) + i, jv);
         S.give(x);
         S.give(y);
       }

@@ -13,6 +13,9 @@ text tables kept under ``profiles/``.
 WRITE_SIZE counter files of a kernel and of the calibration copy into HBM
 bytes per launch (the JSON that bench.py reports as ``roofline.traffic``).
 
+``python -m quatrex_amd.profiling sq <counter_collection.csv>`` prints the
+matrix-core busy fraction and wait fractions per kernel from SQ counters.
+
 ``python -m quatrex_amd.profiling dbstats <results.db>`` prints per-kernel
 call counts and durations from a rocprofv3 ``--kernel-trace`` database.
 
@@ -122,6 +125,34 @@ def summarize(paths: list[str]) -> None:
             for k, (t, n) in sorted(acc.items(), key=lambda kv: -kv[1][0])[:20]:
                 print(f"{k[:70]:70s} {n:7d} {t / 1e6:10.2f} ms {t / n / 1e3:10.2f} us")
         print()
+
+
+def sq_summary(path: str, top: int = 6) -> None:
+    """Derived SQ metrics per kernel from a ``--pmc`` counter file holding
+    SQ_WAVES, SQ_WAVE_CYCLES, SQ_BUSY_CYCLES, SQ_WAIT_INST_ANY,
+    SQ_VALU_MFMA_BUSY_CYCLES, SQ_WAIT_INST_LDS, SQ_INSTS_LDS and one
+    SQ_INSTS_VALU_MFMA_* counter.  SQ_BUSY_CYCLES counts per shader engine
+    (32 on MI355X, 1024 SIMDs), so the SIMD cycles of a kernel are
+    SQ_BUSY_CYCLES x 32 and the matrix-core busy fraction is
+    SQ_VALU_MFMA_BUSY_CYCLES over that; wave-cycle counters are in units of
+    four cycles on both sides of the wait fractions."""
+    acc = defaultdict(lambda: defaultdict(float))
+    disp = defaultdict(set)
+    for r in _rows(Path(path)):
+        acc[r["Kernel_Name"]][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp[r["Kernel_Name"]].add(r.get("Dispatch_Id", ""))
+    order = sorted(acc, key=lambda k: -acc[k].get("SQ_BUSY_CYCLES", 0.0))[:top]
+    print(f"{'kernel':60s} {'disp':>5s} {'mfma busy':>9s} {'wait any':>8s} {'wait lds':>8s} "
+          f"{'mfma/disp':>10s} {'lds/disp':>10s}")
+    for k in order:
+        c = acc[k]
+        simd = c.get("SQ_BUSY_CYCLES", 0.0) * 32
+        wc = c.get("SQ_WAVE_CYCLES", 0.0) or 1.0
+        nd = max(len(disp[k]), 1)
+        mf = sum(v for n, v in c.items() if n.startswith("SQ_INSTS_VALU_MFMA_"))
+        print(f"{k[:60]:60s} {nd:5d} {c.get('SQ_VALU_MFMA_BUSY_CYCLES', 0.0) / max(simd, 1.0):9.3f} "
+              f"{c.get('SQ_WAIT_INST_ANY', 0.0) / wc:8.3f} {c.get('SQ_WAIT_INST_LDS', 0.0) / wc:8.3f} "
+              f"{mf / nd:10.0f} {c.get('SQ_INSTS_LDS', 0.0) / nd:10.0f}")
 
 
 def _per_dispatch(path: Path, kernel: str, counter: str) -> tuple[float, int]:
@@ -245,6 +276,9 @@ def main(argv=None) -> None:
     t.add_argument("--copy-write", required=True)
     t.add_argument("--copy-gib", type=float, default=1.0)
     t.add_argument("--algorithmic-bytes", type=float, default=0.0)
+    q = sub.add_parser("sq")
+    q.add_argument("path")
+    q.add_argument("--top", type=int, default=6)
     d = sub.add_parser("dbstats")
     d.add_argument("path")
     v = sub.add_parser("inv")
@@ -267,6 +301,8 @@ def main(argv=None) -> None:
         import json
 
         print(json.dumps(traffic(args), indent=1))
+    elif args.cmd == "sq":
+        sq_summary(args.path, args.top)
     elif args.cmd == "dbstats":
         dbstats(args.path)
     elif args.cmd == "inv":
