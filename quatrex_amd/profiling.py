@@ -89,6 +89,23 @@ def _rows(path: Path):
         yield from csv.DictReader(f)
 
 
+def _short(name: str) -> str:
+    """Kernel name without the return type, namespaces and argument list."""
+    for pre in ("void ", "qx::(anonymous namespace)::", "qx::"):
+        if name.startswith(pre):
+            name = name[len(pre):]
+    name = name.replace("qx::(anonymous namespace)::", "")
+    depth = 0
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
 def summarize(paths: list[str]) -> None:
     for p in map(Path, paths):
         rows = list(_rows(p))
@@ -98,10 +115,10 @@ def summarize(paths: list[str]) -> None:
         print(f"# {p.name}")
         if "TotalDurationNs" in cols:  # kernel_stats.csv
             tot = sum(float(r["TotalDurationNs"]) for r in rows)
-            print(f"{'kernel':70s} {'calls':>7s} {'total ms':>10s} {'avg us':>10s} {'%':>6s}")
+            print(f"{'kernel':64s} {'calls':>7s} {'total ms':>10s} {'avg us':>10s} {'%':>6s}")
             for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"]))[:20]:
                 t = float(r["TotalDurationNs"])
-                print(f"{r['Name'][:70]:70s} {int(r['Calls']):7d} {t / 1e6:10.2f} "
+                print(f"{_short(r['Name'])[:64]:64s} {int(r['Calls']):7d} {t / 1e6:10.2f} "
                       f"{float(r['AverageNs']) / 1e3:10.2f} {100 * t / tot:6.2f}")
         elif "Counter_Name" in cols:  # counter_collection.csv
             acc = defaultdict(lambda: [0.0, 0])
@@ -142,7 +159,7 @@ def sq_summary(path: str, top: int = 6) -> None:
         acc[r["Kernel_Name"]][r["Counter_Name"]] += float(r["Counter_Value"])
         disp[r["Kernel_Name"]].add(r.get("Dispatch_Id", ""))
     order = sorted(acc, key=lambda k: -acc[k].get("SQ_BUSY_CYCLES", 0.0))[:top]
-    print(f"{'kernel':60s} {'disp':>5s} {'mfma busy':>9s} {'wait any':>8s} {'wait lds':>8s} "
+    print(f"{'kernel':52s} {'disp':>5s} {'mfma busy':>9s} {'wait any':>8s} {'wait lds':>8s} "
           f"{'mfma/disp':>10s} {'lds/disp':>10s}")
     for k in order:
         c = acc[k]
@@ -150,7 +167,7 @@ def sq_summary(path: str, top: int = 6) -> None:
         wc = c.get("SQ_WAVE_CYCLES", 0.0) or 1.0
         nd = max(len(disp[k]), 1)
         mf = sum(v for n, v in c.items() if n.startswith("SQ_INSTS_VALU_MFMA_"))
-        print(f"{k[:60]:60s} {nd:5d} {c.get('SQ_VALU_MFMA_BUSY_CYCLES', 0.0) / max(simd, 1.0):9.3f} "
+        print(f"{_short(k)[:52]:52s} {nd:5d} {c.get('SQ_VALU_MFMA_BUSY_CYCLES', 0.0) / max(simd, 1.0):9.3f} "
               f"{c.get('SQ_WAIT_INST_ANY', 0.0) / wc:8.3f} {c.get('SQ_WAIT_INST_LDS', 0.0) / wc:8.3f} "
               f"{mf / nd:10.0f} {c.get('SQ_INSTS_LDS', 0.0) / nd:10.0f}")
 
