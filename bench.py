@@ -864,6 +864,11 @@ LEG_ARGS = ("n_blocks", "block_size", "energies", "eta", "torch_sample", "eta_en
             "serving_rate", "serving_budget_gib")
 
 
+def _progress(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_leg(leg: str, args, eps: float) -> dict:
     """Run one secondary leg in a child process; its JSON line, or the error."""
     import subprocess
@@ -991,6 +996,7 @@ def main():
         step()
     ctx.status()
     torch.cuda.synchronize()
+    _progress(f"main leg: {args.warmup} warm-up steps done, timing {args.steps}")
     ctx.profile(True)
     ctx.profile_read(reset=True)
     t0 = time.perf_counter()
@@ -1202,7 +1208,10 @@ def main():
     if args.serving_requests > 0:
         legs.append(("serving", "serving"))
     for key, leg in legs:
+        _progress(f"leg {key} ...")
+        t_leg = time.perf_counter()
         out[key] = run_leg(leg, args, eps)
+        _progress(f"leg {key} done in {time.perf_counter() - t_leg:.1f} s")
 
     # speed-ups against the reference algorithm on vendor libraries (rocBLAS
     # ZGEMM + torch.linalg.inv at the reference's default batch of 100)
@@ -1214,6 +1223,7 @@ def main():
             chk = out["mixed"]["with_check_and_fp64_fallback"]
             chk["speedup_vs_torch_fp64"] = round(chk["value"] / tv, 3)
     if not args.no_cpu_baseline:
+        _progress("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(model, energies, args.eta, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
