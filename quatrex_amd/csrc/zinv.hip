@@ -22,13 +22,14 @@
 //   2. S[:, P2] += T1 WA[:, P2]                   (P2 columns only)
 //   3. panel P2 -> T2, WB; the move zeroes rows pi(P2) of T1 as well, and
 //      WB[:, P1] holds those rows of T1
-//   4. WB[:, R] += WB[:, P1] WA[:, R]             (WB's R columns were read
-//                                                   before P1's update)
-//   5. S[:, R] += T1 WA[:, R] + T2 WB[:, R]       (one pass over R)
-//   6. S[:, P1] += T2 WB[:, P1]
+//   4. S[:, P1] += T2 WB[:, P1]  (T1 -> T1')
+//   5. S[:, R] += T1' WA[:, R] + T2 WB[:, R]     (one pass over R)
 // Row by row this is the sequence of the two single-panel updates: rows
 // pi(P1) and pi(P2) are zero in S[:, R] before step 5, and rows pi(P2) of T1
-// are zero there, so their P1 contribution enters only through WB.
+// are zero there, so their P1 contribution enters only through WB.  WB's R
+// columns were moved before P1's update reached them; the missing
+// WB[:, P1] WA[:, R] enters step 5 through T2 WB[:, P1] WA[:, R], which
+// T1' carries (step 4 before step 5).
 // zinv_finish_kernel then writes Out[i][j] = S[pi(i)][pi^{-1}(j)].
 #include <algorithm>
 
@@ -513,27 +514,20 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
     }
     panel(k2, nb2, 1);
     {
+      // S[:, P1] += T2 WB[:, P1]: T1 becomes T1' = T1 + T2 WB[:, P1], which
+      // carries WB's missing P1 contribution into the pass over R below
       ZGroup g{};
       g.nE = nE;
-      const ZTerm a0{wb + k, wa, sAuxC, sAuxC, n, n, kOpN, kOpN, nb1, 0, cmk(1.0, 0.0)};
-      ZTerm a1 = a0;
-      a1.B = wa + k3;
-      add(g, nb2, k, wb, sAuxC, a0, nullptr);
-      add(g, nb2, n - k3, wb + k3, sAuxC, a1, nullptr);
+      add(g, n, nb1, S + k, sS, tS(k2, wb + k, nb2), nullptr);
       if ((err = run(g)) != hipSuccess) return err;
     }
     {
+      // S[:, R] += T1' WA[:, R] + T2 WB[:, R]
       ZGroup g{};
       g.nE = nE;
       const ZTerm l2 = tS(k2, wb, nb2), r2 = tS(k2, wb + k3, nb2);
       add(g, n, k, S, sS, tS(k, wa, nb1), &l2);
       add(g, n, n - k3, S + k3, sS, tS(k, wa + k3, nb1), &r2);
-      if ((err = run(g)) != hipSuccess) return err;
-    }
-    {
-      ZGroup g{};
-      g.nE = nE;
-      add(g, n, nb1, S + k, sS, tS(k2, wb + k, nb2), nullptr);
       if ((err = run(g)) != hipSuccess) return err;
     }
     k = k3;
