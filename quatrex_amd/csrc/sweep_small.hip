@@ -54,7 +54,6 @@ typedef double2 Slot[kB][kLd];
 
 struct SmallShared {
   Slot s[kNS];
-  double2 prow[2][16];  // pivot row of the current elimination step (halves)
   int piv[kB];          // pivot row of each elimination step
   int step_of[kB];      // step at which each row was the pivot
   double red[2][4];
@@ -399,6 +398,31 @@ __device__ inline double readlane_d(double x, int src) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Value of lane `addr / 4` of this wave (per-lane address; ds_bpermute).
+__device__ inline double bpermute_d(double x, int addr) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Pivot of the column whose values lanes of half hc hold in `v` (row l % 32):
+// the unused row of largest key (DPP row maxima of the half's two 16-lane
+// rows); marks it used and returns it with its value (every lane of the
+// wave gets the same).
+__device__ inline int pick_pivot(double2 v, int hc, int n, bool& used, double2& pv) {
+  const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+  unsigned key = (h == hc && r < n && !used) ? pivot_key(v, r) : 0u;
+  key = row_max16(key);
+  const unsigned mx = max((unsigned)__builtin_amdgcn_readlane((int)key, 32 * hc + 15),
+                          (unsigned)__builtin_amdgcn_readlane((int)key, 32 * hc + 31));
+  int p = 31 - (int)(mx & 31u);
+  p = p < n ? p : n - 1;
+  if (r == p) used = true;
+  pv = cmk(readlane_d(v.x, 32 * hc + p), readlane_d(v.y, 32 * hc + p));
+  return p;
+}
+
 // Gauss-Jordan inverse of the n x n valid part of slot ms (identity pad
 // beyond n) with partial pivoting and no row moves, by ONE wave with the
 // matrix in registers (no barrier, so the other waves can work meanwhile):
@@ -409,15 +433,17 @@ __device__ inline double readlane_d(double x, int src) {
 // per step (for free: the update writes register k + 1's new value into
 // register k), so that column c is always register 0 of the lanes of its
 // half and the step loop stays rolled with constant register indices.
-// Step c: those lanes search the column (DPP row maxima of the half's two
-// 16-lane rows), every lane reads the pivot value from the pivot lane, the
-// other half receives its row's column-c value by a lane swap, the pivot
-// row's two lanes publish it in LDS (the wave's LDS accesses complete in
-// order) and every lane applies
+// Step c: every lane fetches its half of the pivot row from the pivot lane
+// (ds_bpermute) and its row's column-c value from the other half if needed,
+// and applies
 //   row <- keep row - g prow,  keep = 1, g = m_rc / pivot  (keep = 0,
 //   g = -1/pivot for the pivot row),
-// then sets column c to -g.  Slot ts is scratch for the permutation back
-// into slot ms.
+// then sets column c to -g.  The step is software-pipelined: register 1
+// (column c + 1) is updated first and the search and reciprocal of step
+// c + 1 run beside the update of the other fifteen registers (one wave has
+// no other work to hide the search's dependent chain behind: 1630 against
+// 1920 cycles per column measured).  Slot ts is scratch for the permutation
+// back into slot ms.
 __device__ __forceinline__ void invert_wave(SmallShared& sh, int ms, int ts, int n, int* info, int tag,
                                             int64_t energy) {
   const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
@@ -425,26 +451,12 @@ __device__ __forceinline__ void invert_wave(SmallShared& sh, int ms, int ts, int
 #pragma unroll
   for (int k = 0; k < 16; ++k) m[k] = sh.s[ms][r][16 * h + k];
   bool used = false;
+  double2 pv;
+  int p = pick_pivot(m[0], 0, n, used, pv);
+  double2 inv = pivot_recip(pv);
 #pragma unroll 1
   for (int c = 0; c < n; ++c) {
     const int hc = c >> 4;
-    // register 0: column c in half hc, column c + 16 - 32 hc in the other
-    unsigned key = (h == hc && r < n && !used) ? pivot_key(m[0], r) : 0u;
-    key = row_max16(key);
-    const unsigned mx = max((unsigned)__builtin_amdgcn_readlane((int)key, 32 * hc + 15),
-                            (unsigned)__builtin_amdgcn_readlane((int)key, 32 * hc + 31));
-    int p = 31 - (int)(mx & 31u);
-    p = p < n ? p : n - 1;
-    const bool me = r == p;
-    if (me) used = true;
-    // publish the pivot row (its two lanes) and fetch the partner half's
-    // column-c value before the reciprocal's dependent chain
-    if (me) {
-#pragma unroll
-      for (int k = 0; k < 16; ++k) sh.prow[h][k] = m[k];
-    }
-    const double ox = __shfl_xor(m[0].x, 32), oy = __shfl_xor(m[0].y, 32);
-    const double2 pv = cmk(readlane_d(m[0].x, 32 * hc + p), readlane_d(m[0].y, 32 * hc + p));
     if (lane == 0) {
       sh.piv[c] = p;
       sh.step_of[p] = c;
@@ -453,26 +465,38 @@ __device__ __forceinline__ void invert_wave(SmallShared& sh, int ms, int ts, int
         info[2] = c;
       }
     }
-    const double2 inv = pivot_recip(pv);
+    const int addr = (32 * h + p) * 4;
+    double2 pr[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) pr[k] = cmk(bpermute_d(m[k].x, addr), bpermute_d(m[k].y, addr));
+    const double ox = __shfl_xor(m[0].x, 32), oy = __shfl_xor(m[0].y, 32);
+    const bool me = r == p;
     const double2 a = (h == hc) ? m[0] : cmk(ox, oy);  // this row's column-c value
     const double2 f = cmul(a, inv);
     const double keep = me ? 0.0 : 1.0;
     const double2 g = me ? cmk(-inv.x, -inv.y) : f;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    asm volatile("" ::: "memory");
     double2 nw[16];
+    // register 1 holds column c + 1 in the half that owns it
+    nw[1] = cmk(keep * m[1].x - (g.x * pr[1].x - g.y * pr[1].y), keep * m[1].y - (g.x * pr[1].y + g.y * pr[1].x));
+    int pn = p;
+    double2 invn = inv, pvn = pv;
+    if (c + 1 < n) {
+      pn = pick_pivot(nw[1], (c + 1) >> 4, n, used, pvn);
+      invn = pivot_recip(pvn);
+    }
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
-      const double2 pr = sh.prow[h][k];
-      nw[k] = cmk(keep * m[k].x - (g.x * pr.x - g.y * pr.y), keep * m[k].y - (g.x * pr.y + g.y * pr.x));
+      if (k == 1) continue;
+      nw[k] = cmk(keep * m[k].x - (g.x * pr[k].x - g.y * pr[k].y), keep * m[k].y - (g.x * pr[k].y + g.y * pr[k].x));
     }
     if (h == hc) nw[0] = cmk(-g.x, -g.y);
     // rotate by one position
 #pragma unroll
     for (int k = 0; k < 15; ++k) m[k] = nw[k + 1];
     m[15] = nw[0];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    asm volatile("" ::: "memory");
+    p = pn;
+    pv = pvn;
+    inv = invn;
   }
   // register k now holds column 16 h + ((k + n) mod 16); permute back
   // through slot ts: Out[i][j] = M[pi(i)][pi^-1(j)]
