@@ -577,32 +577,21 @@ int sweep_chunk_solve(Ctx* c, const Geo& g, int64_t e0, int E, const SolveIO& io
       if (rc) return rc;
       rc = c->gemm({pYl, pYg, pQl, pQg}, E, bj);
       if (rc) return rc;
-    } else if (!c->overlap) {
+    } else {
       // P = A_ji g_i ; Y = A_ji g^<>_i
       rc = c->gemm({pP, pYl, pYg}, E, bj);
       if (rc) return rc;
       // S = A_jj - B^R_j - P A_ij ; Q = P Sigma_ij
       rc = c->gemm({pS, pQl, pQg}, E, bj);
       if (rc) return rc;
-    } else {
-      // P, then S alone: the inversion of S can then start (side stream)
-      // while Y, Q and W, none of which needs g_j, run on the main stream
-      rc = c->gemm({pP}, E, bj);
-      if (!rc) rc = c->gemm({pS}, E, bj);
-      if (rc) return rc;
     }
-    // g_j = S^{-1} is independent of Y, Q and W: with overlap it runs on the
-    // side stream
+    // g_j = S^{-1} is independent of W: with overlap it runs on the side stream
     const bool ovl = c->overlap;
     if (ovl) {
       rc = c->fork();
       if (rc) return rc;
       rc = c->inv(S.p, S.s, bj, E, GR.D(j), w.aux, j + 1, e0, c->side);
       if (rc) return rc;
-      if (!(c->prec32 && c->refine < 0)) {
-        rc = c->gemm({pYl, pYg, pQl, pQg}, E, bj);
-        if (rc) return rc;
-      }
     }
     // W = Y A_ji^H + Q^H - Q  [-> Sigma~_j]  + Sigma_jj + B_j
     rc = c->gemm({PB(bj, bj)
