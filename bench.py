@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
-    ap.add_argument("--no-autotune", action="store_true", help="default 64x64 GEMM tiles")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="untuned GEMM tiles (FP64: 3M 64x64, 3M 32x32 below 64; FP32: 64x64)")
     ap.add_argument("--tile", default=None,
                     help="FP64 GEMM tile configuration of the main leg (skips its autotuning)")
     ap.add_argument("--torch-sample", type=int, default=100,
@@ -977,7 +978,8 @@ def main():
     from quatrex_amd import _lib
     from quatrex_amd import autotune as AT
 
-    tiles = {"f64": "64x64", "f32": "64x64"}
+    tiles = {"f64": "64x64-3m" if args.block_size >= 64 else "32x32-3m",
+             "f32": "64x64" if args.block_size >= 64 else "32x32"}
     if args.tile:
         tiles["f64"] = args.tile
         ctx.set_option(_lib.RGF_OPT_TILE_FOR_BLOCK + args.block_size, _lib.TILE_CFG_NAMES.index(args.tile))

@@ -106,11 +106,18 @@ int Ctx::ensure_meta(size_t bytes) {
   return RGF_OK;
 }
 
+// Untuned tile: the one the autotuner picks at production batch sizes on
+// MI355X (3M FP64 products; the FP32 products have no 3M variant).
+static int default_tile(int b, bool f32) {
+  if (f32) return b < 64 ? kTile32x32 : kTile64x64;
+  return b < 64 ? kTile32x32_3M : kTile64x64_3M;
+}
+
 int Ctx::tile_for(int b) const {
   if (tile_override >= 0) return tile_override;
   const std::vector<int>& t = prec32 ? tile_table32 : tile_table;
   if (b >= 0 && b < (int)t.size() && t[b] >= 0) return t[b];
-  return b < 64 ? kTile32x32 : kTile64x64;
+  return default_tile(b, prec32);
 }
 
 // ---------------------------------------------------------------------------
@@ -1703,7 +1710,7 @@ int rgf_autotune(rgf_ctx* ctx, int64_t b, int64_t n_energies, int32_t f32, int32
   ctx->profile = false;
   // Configurations are timed round-robin over several rounds (minimum per
   // configuration), so clock drift cannot favour one; another configuration
-  // replaces the 64x64 default only when it is at least 2% faster.
+  // replaces the untuned default only when it is at least 2% faster.
   float tmin[qx::kNumTileCfgs];
   for (int cfg = 0; cfg < qx::kNumTileCfgs; ++cfg) tmin[cfg] = -1.f;
   for (int round = 0; round < 5 && !rc; ++round) {
@@ -1726,8 +1733,8 @@ int rgf_autotune(rgf_ctx* ctx, int64_t b, int64_t n_energies, int32_t f32, int32
     if (ms_per_cfg) ms_per_cfg[cfg] = tmin[cfg];
     if (tmin[cfg] >= 0.f && tmin[cfg] < best) { best = tmin[cfg]; best_c = cfg; }
   }
-  if (best_c != qx::kTile64x64 && tmin[qx::kTile64x64] >= 0.f && best > 0.98f * tmin[qx::kTile64x64])
-    best_c = qx::kTile64x64;
+  const int def_c = qx::default_tile((int)b, f32 != 0);
+  if (best_c != def_c && tmin[def_c] >= 0.f && best > 0.98f * tmin[def_c]) best_c = def_c;
   ctx->prec32 = saved32;
   ctx->tile_override = saved_over;
   ctx->profile = saved_prof;
