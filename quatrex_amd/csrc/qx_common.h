@@ -29,6 +29,38 @@ __host__ __device__ inline double2 crecip(double2 d) {
   return cmk(xr / den, -xi / den);
 }
 
+// 1/d for a pivot: conj(d) / |d|^2 with the reciprocal of |d|^2 from
+// v_rcp_f64 and two Newton steps; outside the range where |d|^2 neither
+// overflows nor loses precision the scaled division of crecip.  Callers pass
+// the same d in every thread, so the branch is uniform.
+static __device__ __noinline__ double2 crecip_slow(double2 d) { return crecip(d); }
+static __device__ inline double2 pivot_recip(double2 d) {
+  const double m = d.x * d.x + d.y * d.y;
+  if (!(m > 1e-280 && m < 1e280)) return crecip_slow(d);
+  double r = __builtin_amdgcn_rcp(m);
+  r = fma(r, fma(-m, r, 1.0), r);
+  r = fma(r, fma(-m, r, 1.0), r);
+  return cmk(d.x * r, -d.y * r);
+}
+// The same with the out-of-range case inline (no call: a call makes the
+// caller save its live registers around it).
+static __device__ inline double2 pivot_recip_inline(double2 d) {
+  const double m = d.x * d.x + d.y * d.y;
+  if (!(m > 1e-280 && m < 1e280)) return crecip(d);
+  double r = __builtin_amdgcn_rcp(m);
+  r = fma(r, fma(-m, r, 1.0), r);
+  r = fma(r, fma(-m, r, 1.0), r);
+  return cmk(d.x * r, -d.y * r);
+}
+
+// Value of lane `src` (uniform) of this wave.
+static __device__ inline double readlane_d(double x, int src) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 enum Op : int32_t { kOpN = 0, kOpC = 1 };
 enum D2Mode : int32_t { kD2None = 0, kD2Partial = 1, kD2Copy = 2, kD2ConjT = 3 };
 

@@ -357,20 +357,6 @@ __device__ inline void store_blk(double2* dst, int ld, int rows, int cols, const
   }
 }
 
-// 1/d for the pivot: conj(d) / |d|^2 with the reciprocal of |d|^2 from
-// v_rcp_f64 and two Newton steps; outside the range where |d|^2 neither
-// overflows nor loses precision the scaled division of crecip.  Every thread
-// passes the same d, so the branch is uniform.
-__device__ __noinline__ double2 crecip_slow(double2 d) { return crecip(d); }
-__device__ inline double2 pivot_recip(double2 d) {
-  const double m = d.x * d.x + d.y * d.y;
-  if (!(m > 1e-280 && m < 1e280)) return crecip_slow(d);
-  double r = __builtin_amdgcn_rcp(m);
-  r = fma(r, fma(-m, r, 1.0), r);
-  r = fma(r, fma(-m, r, 1.0), r);
-  return cmk(d.x * r, -d.y * r);
-}
-
 // Pivot key of a candidate row: the high word of |re| + |im| (non-negative
 // doubles order like their bit patterns; NaN above infinity), with its low six
 // bits replaced by a marker bit (every candidate key is nonzero) and 31 - row
@@ -388,14 +374,6 @@ __device__ inline unsigned row_max16(unsigned v) {
   v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x114, 0xf, 0xf, false));
   v = max(v, (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x118, 0xf, 0xf, false));
   return v;
-}
-
-// Value of lane `src` (uniform) of this wave.
-__device__ inline double readlane_d(double x, int src) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 // Value of lane `addr / 4` of this wave (per-lane address; ds_bpermute).

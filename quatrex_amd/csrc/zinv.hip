@@ -8,9 +8,16 @@
 // [k, k+nb) is eliminated by zinv_panel_kernel (one workgroup per energy, one
 // physical row per thread, the row's panel part in registers; for matrices up
 // to 32 rows zinv_panel2_kernel, two threads per row): unblocked GJE on the
-// panel with the pivot search by wavefront shuffle max-reduction.  It
-// writes the transformed panel T in place, moves the pivot rows' non-panel
-// columns to Wp (nb x n) and zeroes them in S.  The blocked GJE update is
+// panel.  In zinv_panel_kernel a step takes one barrier: before it every
+// wave publishes its best candidate row (DPP maximum of the key) and the
+// key; after it every thread takes the winner and applies
+//   row <- keep row + g prow   (keep = 1, g = -a_r / a_p; for the pivot row
+//   keep = 0, g = 1 / a_p)
+// with the unscaled pivot row prow, a_r the row's pivot-column value and a_p
+// the pivot, so no thread scales the pivot row alone while the others wait.
+// The panel kernels write the transformed panel T in place, move the pivot
+// rows' non-panel columns to Wp (nb x n) and zero them in S.  The blocked
+// GJE update is
 //   S[:, nonpanel] += T * Wp[:, nonpanel],
 // i.e. S += (T - J) Wp with J[pi(k+q)][q] = 1: J Wp restores the pivot rows,
 // which the move to Wp left at zero.
@@ -24,6 +31,7 @@
 //      WB[:, P1] holds those rows of T1
 //   4. S[:, P1] += T2 WB[:, P1]  (T1 -> T1')
 //   5. S[:, R] += T1' WA[:, R] + T2 WB[:, R]     (one pass over R)
+// (2 and 4 are K = nb updates of nb columns, run on 32 x 32 tiles.)
 // Row by row this is the sequence of the two single-panel updates: rows
 // pi(P1) and pi(P2) are zero in S[:, R] before step 5, and rows pi(P2) of T1
 // are zero there, so their P1 contribution enters only through WB.  WB's R
@@ -58,6 +66,25 @@ struct PanelAux {
 
 inline int panel_nb(int n) { return n <= 512 ? 32 : 16; }
 
+// Maximum over each row of 16 lanes, left in the row's last lane (row_shr
+// 1, 2, 4, 8 on both halves of the double; a lane whose source lies before its
+// row keeps its own value).
+template <int CTRL>
+__device__ inline double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp((int)(b & 0xffffffffll), (int)(b & 0xffffffffll), CTRL,
+                                             0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline double row_max16_d(double v) {
+  v = fmax(v, dpp_d<0x111>(v));
+  v = fmax(v, dpp_d<0x112>(v));
+  v = fmax(v, dpp_d<0x114>(v));
+  v = fmax(v, dpp_d<0x118>(v));
+  return v;
+}
+
 // Shared state of one panel workgroup.
 // Panel rows are moved between HBM and registers through LDS in chunks of
 // kChunk columns, so that global accesses are row-contiguous.  That staging
@@ -71,6 +98,14 @@ struct PanelShared {
   double red_key[16];
   int red_idx[16];
   double2 piv_row[NB];
+  int s_piv[NB];
+};
+// The one-thread-per-row panel kernel: per wave its candidate pivot row and
+// (key, row), both double-buffered by step.
+template <int NB, int MAXW>
+struct PanelShared1 {
+  double2 cand[2][MAXW][NB];
+  double2 red[2][16];
   int s_piv[NB];
 };
 
@@ -89,7 +124,7 @@ zinv_panel_kernel(double2* S, int64_t sS, int n, int k, int nbe, char* aux, int6
   int* piv = reinterpret_cast<int*>(ax + PanelAux<NB>::piv(n));
   int* step_of = reinterpret_cast<int*>(ax + PanelAux<NB>::step(n));
 
-  __shared__ PanelShared<NB> sh;
+  __shared__ PanelShared1<NB, MAXT / 64> sh;
   extern __shared__ __align__(16) double2 stage_raw[];
   auto stage = reinterpret_cast<double2(*)[kChunk + 1]>(stage_raw);
 
@@ -105,90 +140,141 @@ zinv_panel_kernel(double2* S, int64_t sS, int n, int k, int nbe, char* aux, int6
   // the (rolled) loop body indexes registers with constants only.
   double re[NB], im[NB];
   const int rows = blockDim.x;
-#pragma unroll
-  for (int q0 = 0; q0 < NB; q0 += kChunk) {
-    // rows x kChunk tile, kChunk consecutive lanes per row segment
+  // rows x kChunk tiles, kChunk consecutive lanes per row segment
+  // (coalesced); thread tid moves column cc of rows rr0, rr0 + rstep, ...
+  // (rows is a multiple of kChunk).  The next chunk's global loads are in
+  // flight while the current one passes through LDS.
+  const int cc = tid % kChunk, rr0 = tid / kChunk, rstep = rows / kChunk;
+  double2 ld[2][kChunk];
+  auto fetch = [&](int q0, double2* dst) {
+    const bool col_ok = q0 + cc < nbe;
+    const double2* src = A + (int64_t)rr0 * n + k + q0 + cc;
 #pragma unroll
     for (int i = 0; i < kChunk; ++i) {
-      const int idx = tid + i * rows;
-      const int rr = idx / kChunk, q = q0 + idx % kChunk;
-      stage[rr][idx % kChunk] = (rr < n && q < nbe) ? A[(int64_t)rr * n + k + q] : cmk(0.0, 0.0);
+      const int rr = rr0 + i * rstep;
+      dst[i] = (rr < n && col_ok) ? src[(int64_t)i * rstep * n] : cmk(0.0, 0.0);
     }
+  };
+  fetch(0, ld[0]);
+#pragma unroll
+  for (int c = 0; c < NB / kChunk; ++c) {
+    if (c + 1 < NB / kChunk) fetch((c + 1) * kChunk, ld[(c + 1) & 1]);
+#pragma unroll
+    for (int i = 0; i < kChunk; ++i) stage[rr0 + i * rstep][cc] = ld[c & 1][i];
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kChunk; ++q) {
       const double2 v = stage[r][q];
-      re[q0 + q] = v.x;
-      im[q0 + q] = v.y;
+      re[c * kChunk + q] = v.x;
+      im[c * kChunk + q] = v.y;
     }
     __syncthreads();
+  }
+
+  // Wave maximum of the pivot key of the current pivot column (position 0)
+  // and the lane of its row (lowest on ties; -1 when the wave has no
+  // candidate row left): DPP maxima of the four 16-lane rows.
+  auto wave_pick = [&](double& wmax, int& cl) {
+    double key = (valid && step < 0) ? fabs(re[0]) + fabs(im[0]) : -1.0;
+    key = (key != key) ? HUGE_VAL : key;  // NaN: keep the order total
+    const double v = row_max16_d(key);
+    wmax = fmax(fmax(readlane_d(v, 15), readlane_d(v, 31)),
+                fmax(readlane_d(v, 47), readlane_d(v, 63)));
+    cl = (int)__builtin_ctzll(__ballot(key == wmax));
+  };
+  // Every wave publishes its candidate row (unscaled) and (key, row) before
+  // the step's one barrier, double-buffered by step; after it every thread
+  // takes the winner (first wave on ties: rows ascend with the wave).
+  auto publish = [&](int b, double wmax, int cl) {
+    if (lane == cl) {
+      // as 8-byte halves (ds_write2_b64 takes them from any registers)
+      double* ps = reinterpret_cast<double*>(sh.cand[b][wave]);
+#pragma unroll
+      for (int q = 0; q < NB; ++q) {
+        ps[2 * q] = re[q];
+        ps[2 * q + 1] = im[q];
+      }
+    }
+    if (lane == 0) sh.red[b][wave] = cmk(wmax, (double)(wave * 64 + cl));
+  };
+  {
+    double wmax;
+    int cl;
+    wave_pick(wmax, cl);
+    publish(0, wmax, cl);
   }
 
 #pragma unroll 1
   for (int jj = 0; jj < nbe; ++jj) {
     const int c = k + jj;
-    double key = (valid && step < 0) ? fabs(re[0]) + fabs(im[0]) : -1.0;
-    key = (key != key) ? HUGE_VAL : key;  // NaN: keep the order total
-    // wavefront max of the key, then the lowest lane holding it
-    double wmax = key;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
-    const unsigned long long hit = __ballot(key == wmax);
-    if (lane == 0) {
-      sh.red_key[wave] = wmax;
-      sh.red_idx[wave] = wave * 64 + (int)__builtin_ctzll(hit);
-    }
+    const int b = jj & 1;
     __syncthreads();
-    double bk = sh.red_key[0];
-    int p = sh.red_idx[0];
-    for (int w = 1; w < nw; ++w) {
-      const double ok = sh.red_key[w];
-      if (ok > bk) { bk = ok; p = sh.red_idx[w]; }  // waves in row order: first wins ties
+    int p, wb;
+    {
+      // lane w < nw holds wave w's entry
+      const double2 ent = sh.red[b][lane & 15];
+      const double key = (lane < nw) ? ent.x : -2.0;
+      const double v = row_max16_d(key);
+      const double mx = readlane_d(v, 15);
+      wb = (int)__builtin_ctzll(__ballot(key == mx) & 0xffffull);
+      p = (int)readlane_d(ent.y, wb);
+      p = (p < n) ? p : n - 1;  // memory safety only; a candidate row always wins
     }
-    p = (p < n) ? p : n - 1;  // memory safety only; a candidate row always wins
-    const bool me = (r == p);
-    if (me) {
-      // the pivot thread publishes its scaled row (position 0: the pivot
-      // column, which becomes 1/pivot)
-      if (re[0] == 0.0 && im[0] == 0.0) {
-        if (atomicCAS(info, 0, info_tag) == 0) {
-          info[1] = (int)(e0 + e);
-          info[2] = c;
+    const double2* prow = sh.cand[b][wb];
+    const double2 pa = prow[0];
+    if (tid == 0) {
+      sh.s_piv[jj] = p;
+      if (pa.x == 0.0 && pa.y == 0.0 && atomicCAS(info, 0, info_tag) == 0) {
+        info[1] = (int)(e0 + e);
+        info[2] = c;
+      }
+    }
+    const bool me = r == p;
+    if (me) step = c;
+    // new[q - 1] = keep row[q] + g prow[q] (q >= 1), new[NB - 1] = g, with
+    // keep = 1, g = -a_r / a_p for the other rows and keep = 0, g = 1 / a_p
+    // for the pivot row (a_r: the row's pivot-column value, a_p: the pivot);
+    // pivot-row values in groups of four, the next group's reads issued
+    // before the current group's update
+    constexpr int G = 4, NG = (NB - 1 + G - 1) / G;
+    double2 pv[2][G];
+    auto load = [&](int grp, double2* dst) {
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int q = 1 + G * grp + i;
+        if (q < NB) dst[i] = prow[q];
+      }
+    };
+    load(0, pv[0]);
+    const double2 inv = pivot_recip_inline(pa);
+    const double gr = me ? inv.x : -(re[0] * inv.x - im[0] * inv.y);
+    const double gi = me ? inv.y : -(re[0] * inv.y + im[0] * inv.x);
+    const double keep = me ? 0.0 : 1.0;
+    double wmax = -1.0;
+    int cl = 0;
+#pragma unroll
+    for (int grp = 0; grp < NG; ++grp) {
+      if (grp + 1 < NG) load(grp + 1, pv[(grp + 1) & 1]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int q = 1 + G * grp + i;
+        if (q < NB) {
+          const double2 v = pv[grp & 1][i];
+          re[q - 1] = fma(gr, v.x, fma(-gi, v.y, keep * re[q]));
+          im[q - 1] = fma(gr, v.y, fma(gi, v.x, keep * im[q]));
         }
       }
-      const double2 inv = crecip(cmk(re[0], im[0]));
-      sh.piv_row[0] = inv;
-#pragma unroll
-      for (int q = 1; q < NB; ++q)
-        sh.piv_row[q] = cmk(re[q] * inv.x - im[q] * inv.y, re[q] * inv.y + im[q] * inv.x);
-      step = c;
+      // the next step's pivot search (position 0 is final after the first
+      // group) beside the second group's update
+      if (grp == 1) wave_pick(wmax, cl);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (tid == 0) sh.s_piv[jj] = p;
-    __syncthreads();
-    // new[q] = keep * row[q] - f * prow[q] (row[0] counts as 0), with
-    // keep = 0, f = -1 for the pivot row (new = prow); then rotate by one.
-    const double keep = me ? 0.0 : 1.0;
-    const double fr = me ? -1.0 : re[0], fi = me ? 0.0 : im[0];
-    const double2 p0 = sh.piv_row[0];
-    const double n0r = -(fr * p0.x - fi * p0.y);
-    const double n0i = -(fr * p0.y + fi * p0.x);
-#pragma unroll
-    for (int q0 = 1; q0 < NB; q0 += 8) {
-      // groups of 8 pivot-row values: keeps the LDS reads from all being
-      // hoisted at once
-      asm volatile("" ::: "memory");
-#pragma unroll
-      for (int q = q0; q < q0 + 8 && q < NB; ++q) {
-        const double2 pv = sh.piv_row[q];
-        const double nr = keep * re[q] - (fr * pv.x - fi * pv.y);
-        const double ni = keep * im[q] - (fr * pv.y + fi * pv.x);
-        re[q - 1] = nr;
-        im[q - 1] = ni;
-      }
-    }
-    re[NB - 1] = n0r;
-    im[NB - 1] = n0i;
+    re[NB - 1] = gr;  // the eliminated column rotates into the last position
+    im[NB - 1] = gi;
+    publish(b ^ 1, wmax, cl);
   }
+  __syncthreads();
 
   // after nbe rotations position q holds panel column (q + nbe) mod NB
   if (valid) step_of[r] = step;
@@ -491,6 +577,12 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
     P.D = D; P.sD = sD; P.ldd = n;
   };
   auto run = [&](ZGroup& g) { return g.nprob ? launch_zgemm_update(g, tile_cfg, stream) : hipSuccess; };
+  // the narrow K = NB updates of a pair's panel columns on full 32 x 32 tiles
+  // (a 64 x 64 tile would be half empty; 512 inverses of 256 x 256: 3.16 ->
+  // 3.03 ms, the 3M 32 x 32 tile 3.10 ms)
+  auto run_narrow = [&](ZGroup& g) {
+    return g.nprob ? launch_zgemm_update(g, kTile32x32, stream) : hipSuccess;
+  };
   hipError_t err = hipSuccess;
   for (int k = 0; k < n;) {
     const int nb1 = (n - k < NB) ? (n - k) : NB;
@@ -510,7 +602,7 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
       ZGroup g{};
       g.nE = nE;
       add(g, n, nb2, S + k2, sS, tS(k, wa + k2, nb1), nullptr);
-      if ((err = run(g)) != hipSuccess) return err;
+      if ((err = run_narrow(g)) != hipSuccess) return err;
     }
     panel(k2, nb2, 1);
     {
@@ -519,7 +611,7 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
       ZGroup g{};
       g.nE = nE;
       add(g, n, nb1, S + k, sS, tS(k2, wb + k, nb2), nullptr);
-      if ((err = run(g)) != hipSuccess) return err;
+      if ((err = run_narrow(g)) != hipSuccess) return err;
     }
     {
       // S[:, R] += T1' WA[:, R] + T2 WB[:, R]

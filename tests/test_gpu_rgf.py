@@ -35,7 +35,8 @@ def _t(x, device):
 # --------------------------------------------------------------------------- inverse
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 17, 32, 33, 64, 100, 128, 256, 300, 512, 640])
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 32, 33, 64, 65, 100, 128, 129, 200, 255, 256, 300, 512,
+                               640])
 def test_batched_inverse_vs_numpy(device, n):
     rng = np.random.default_rng(n)
     nm = 3
@@ -62,6 +63,21 @@ def test_inverse_needs_pivoting(device):
     out = S.inv(_t(a, device)).cpu().numpy()
     S.default_context(device).status()
     assert np.abs(out @ a - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [4, 48, 96, 200, 300])
+def test_singular_inverse_raises(device, n):
+    """A zero row: the last elimination step meets an exactly zero pivot, in
+    every panel layout (two threads per row, 64-column panels, paired
+    32-column panels)."""
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((2, n, n)) + 1j * rng.standard_normal((2, n, n))
+    a[1, n // 3, :] = 0.0
+    ctx = S.default_context(device)
+    S.inv(_t(a, device))
+    with pytest.raises(np.linalg.LinAlgError):
+        ctx.status()
+    ctx.status()  # status resets after reporting
 
 
 def test_singular_block_raises(device):
