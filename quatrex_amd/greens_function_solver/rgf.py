@@ -26,6 +26,8 @@ blocks between the caller's containers and packed device buffers.
 
 from __future__ import annotations
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from quatrex_amd import solve as _solve
@@ -175,6 +177,122 @@ def _zeros_current(like_container, shape):
     return np.zeros(shape, dtype=np.complex128)
 
 
+# Input bytes per transfer chunk of the host-resident pipeline: large enough
+# that each chunk's solve keeps the GPU busy, small enough that the staging
+# of all slots stays modest (4 energies of 32 x 256: ~0.9 GB in, ~1.2 GB out).
+_HOST_CHUNK_BYTES = 1 << 30
+# Staging slots: chunks in flight between the host copies, the transfers and
+# the solves.
+_HOST_SLOTS = 3
+
+
+def _host_packed(m) -> bool:
+    """A host-resident BlockTridiagonal whose packed storage can be copied
+    part by part (non-symmetric, complex128, one stack axis)."""
+    torch = _torch()
+    return (isinstance(m, BlockTridiagonal) and m.device.type == "cpu" and not m.symmetry
+            and m.dtype == torch.complex128 and len(m.stack_shape) == 1
+            and all(t.is_contiguous() for t in m.packed(flatten=False)))
+
+
+class _HostPipe:
+    """Staging of one selected-solve batch of host-resident stacks in energy
+    chunks, so that the host copies, the PCIe transfers and the solves of
+    consecutive chunks overlap.
+
+    Per chunk c (slot s = c % slots): the host copies the inputs from the
+    caller's pageable tensors into pinned buffers s; a copy stream moves them
+    to device buffers s; the solve runs on the context stream into device
+    outputs s; a second copy stream moves those to pinned outputs s; a worker
+    thread copies them into the caller's outputs while the main thread stages
+    the next chunks' inputs.  Events order the reuse of every buffer `slots`
+    chunks later.  Results are those of one solve over the batch: energies
+    are independent.
+    """
+
+    def __init__(self, device, in_sizes, out_sizes, chunk, slots):
+        torch = _torch()
+        z = torch.complex128
+        self.chunk, self.slots = chunk, slots
+        self.in_sizes, self.out_sizes = in_sizes, out_sizes
+        S = range(slots)
+        self.pin_in = [[torch.empty((chunk, n), dtype=z, pin_memory=True) for n in in_sizes] for _ in S]
+        self.dev_in = [[torch.empty((chunk, n), dtype=z, device=device) for n in in_sizes] for _ in S]
+        self.pin_out = [[torch.empty((chunk, n), dtype=z, pin_memory=True) for n in out_sizes] for _ in S]
+        self.dev_out = [[torch.empty((chunk, n), dtype=z, device=device) for n in out_sizes] for _ in S]
+        self.copy_in = torch.cuda.Stream(device)
+        self.copy_out = torch.cuda.Stream(device)
+        self.ev_in = [torch.cuda.Event() for _ in S]
+        self.ev_solve = [torch.cuda.Event() for _ in S]
+        self.ev_out = [torch.cuda.Event() for _ in S]
+        self.pool = ThreadPoolExecutor(max_workers=1)
+
+    def matches(self, in_sizes, out_sizes, chunk, slots) -> bool:
+        return (self.in_sizes, self.out_sizes, self.chunk, self.slots) == (in_sizes, out_sizes, chunk, slots)
+
+    def run(self, host_in, host_out, E, solve, stream):
+        """host_in / host_out: lists of (E, n) host tensors in the order of
+        in_sizes / out_sizes; solve(c0, c1, dev_in, dev_out) enqueues the
+        solve of energies [c0, c1) on ``stream`` with device buffers holding
+        them in their first c1 - c0 rows."""
+        torch = _torch()
+        C, S = self.chunk, self.slots
+        nch = (E + C - 1) // C
+        rng = [(c * C, min(E, (c + 1) * C)) for c in range(nch)]
+
+        def stage_in(c):
+            s, (c0, c1) = c % S, rng[c]
+            if c >= S:
+                self.ev_in[s].synchronize()  # pinned inputs s no longer read
+            for pin, src in zip(self.pin_in[s], host_in):
+                pin[: c1 - c0].copy_(src[c0:c1])
+            with torch.cuda.stream(self.copy_in):
+                if c >= S:
+                    self.copy_in.wait_event(self.ev_solve[s])  # device inputs s no longer read
+                for dev, pin in zip(self.dev_in[s], self.pin_in[s]):
+                    dev[: c1 - c0].copy_(pin[: c1 - c0], non_blocking=True)
+                self.ev_in[s].record(self.copy_in)
+
+        def solve_chunk(c):
+            s, (c0, c1) = c % S, rng[c]
+            stream.wait_event(self.ev_in[s])
+            if c >= S:
+                stream.wait_event(self.ev_out[s])  # device outputs s copied out
+            solve(c0, c1, [t[: c1 - c0] for t in self.dev_in[s]], [t[: c1 - c0] for t in self.dev_out[s]])
+            self.ev_solve[s].record(stream)
+            with torch.cuda.stream(self.copy_out):
+                self.copy_out.wait_event(self.ev_solve[s])
+                for pin, dev in zip(self.pin_out[s], self.dev_out[s]):
+                    pin[: c1 - c0].copy_(dev[: c1 - c0], non_blocking=True)
+                self.ev_out[s].record(self.copy_out)
+
+        def stage_out(c):
+            s, (c0, c1) = c % S, rng[c]
+            self.ev_out[s].synchronize()
+            for dst, pin in zip(host_out, self.pin_out[s]):
+                dst[c0:c1].copy_(pin[: c1 - c0])
+
+        # the copies out of pinned memory run on the worker thread beside the
+        # copies into it (torch releases the GIL in copy_); pinned outputs s
+        # are free for chunk c's transfer once chunk c - S's copy is done
+        pending = [None] * S
+        for c in range(min(S - 1, nch)):
+            stage_in(c)
+        for c in range(nch):
+            if pending[c % S] is not None:
+                pending[c % S].result()
+                pending[c % S] = None
+            solve_chunk(c)
+            if c + S - 1 < nch:
+                stage_in(c + S - 1)
+            if c >= 1:
+                pending[(c - 1) % S] = self.pool.submit(stage_out, c - 1)
+        stage_out(nch - 1)
+        for f in pending:
+            if f is not None:
+                f.result()
+
+
 class RGF(GFSolver):
     """Selected inversion / selected solve by the recursive Green's function
     method on MI355X.
@@ -234,6 +352,48 @@ class RGF(GFSolver):
         return None
 
     # ------------------------------------------------------------------
+    def _solve_host_batch(self, geo, ctx, device, sl, a, sigma_lesser, sigma_greater, xl, xg, xr,
+                          obc_blocks, current) -> bool:
+        """One energy batch of host-resident stacks through ``_HostPipe``;
+        False (nothing done) when the batch is too small to split."""
+        a_, sl_, sg_ = a.stack[sl], sigma_lesser.stack[sl], sigma_greater.stack[sl]
+        E = int(a_.stack_shape[0])
+        host_in = list(a_.packed()) + list(sl_.packed()[:2]) + list(sg_.packed()[:2])
+        outs = [xl.stack[sl], xg.stack[sl]] + ([xr.stack[sl]] if xr is not None else [])
+        host_out = [t for o in outs for t in o.packed()]
+        in_sizes = tuple(int(t.shape[1]) for t in host_in)
+        out_sizes = tuple(int(t.shape[1]) for t in host_out)
+        chunk = max(1, min((E + 1) // 2, _HOST_CHUNK_BYTES // (16 * sum(in_sizes))))
+        if E < 2:
+            return False
+        pipe = getattr(self, "_pipe", None)
+        if pipe is None or not pipe.matches(in_sizes, out_sizes, chunk, _HOST_SLOTS):
+            self._pipe = None  # release the old staging first
+            pipe = self._pipe = _HostPipe(device, in_sizes, out_sizes, chunk, _HOST_SLOTS)
+        has_obc = any(t is not None for t in obc_blocks.retarded + obc_blocks.lesser + obc_blocks.greater)
+        torch = _torch()
+        cur = (torch.empty((E, geo.n_blocks + 1), dtype=torch.complex128, device=device)
+               if current is not None else None)
+
+        def solve(c0, c1, din, dout):
+            n = c1 - c0
+            obc = None
+            if has_obc:
+                csl = slice(sl.start + c0, sl.start + c1)
+                obc = (_obc_batch(obc_blocks.retarded, csl, geo, n, device),
+                       _obc_batch(obc_blocks.lesser, csl, geo, n, device),
+                       _obc_batch(obc_blocks.greater, csl, geo, n, device))
+            _solve.selected_solve(geo, tuple(din[0:3]), (din[3], din[4]), (din[5], din[6]),
+                                  g_lesser=tuple(dout[0:3]), g_greater=tuple(dout[3:6]),
+                                  g_retarded=tuple(dout[6:9]) if xr is not None else None, obc=obc,
+                                  current=cur[c0:c1] if cur is not None else None, ctx=ctx)
+
+        pipe.run(host_in, host_out, E, solve, ctx.stream)
+        ctx.status()
+        if cur is not None:
+            current[sl] = cur if not isinstance(current, np.ndarray) else cur.cpu().numpy()
+        return True
+
     def selected_solve(self, a, sigma_lesser, sigma_greater, obc_blocks: OBCBlocks | None = None,
                        out=None, return_retarded: bool = False, return_current: bool = False):
         device = self._dev()
@@ -261,8 +421,15 @@ class RGF(GFSolver):
             if return_retarded:
                 xr = a.__class__.zeros_like(a)
 
+        host = (_host_packed(a) and _host_packed(sigma_lesser) and _host_packed(sigma_greater)
+                and _host_packed(xl) and _host_packed(xg) and (not return_retarded or _host_packed(xr)))
         for bidx in range(len(offs) - 1):
             sl = slice(int(offs[bidx]), int(offs[bidx + 1]))
+            if host and self._solve_host_batch(geo, ctx, device, sl, a, sigma_lesser, sigma_greater,
+                                               xl, xg, xr if return_retarded else None, obc_blocks,
+                                               current if return_current else None):
+                continue
+
             def inp(m, parts):
                 if _is_dsdb(m):
                     return _gather_dsdb(m, sl, geo, parts, device, ctx)
