@@ -327,8 +327,11 @@ class RGF(GFSolver):
             obc_blocks = OBCBlocks(num_blocks=a.num_blocks)
         _, offs = get_batches(a.shape[0], self.max_batch_size)
         x = out if out is not None else a.__class__.zeros_like(a)
+        host = _host_packed(a) and _host_packed(x)
         for bidx in range(len(offs) - 1):
             sl = slice(int(offs[bidx]), int(offs[bidx + 1]))
+            if host and self._inv_host_batch(geo, ctx, device, sl, a, x, obc_blocks):
+                continue
             if _is_dsdb(a):
                 A, lead = _gather_dsdb(a, sl, geo, "dul", device, ctx)
             else:
@@ -352,6 +355,40 @@ class RGF(GFSolver):
         return None
 
     # ------------------------------------------------------------------
+    def _pipe_for(self, device, in_sizes, out_sizes, E):
+        """The staging of this solver for a batch of E host-resident energies
+        (None when the batch is too small to split)."""
+        chunk = max(1, min((E + 1) // 2, _HOST_CHUNK_BYTES // (16 * sum(in_sizes))))
+        if E < 2:
+            return None
+        pipe = getattr(self, "_pipe", None)
+        if pipe is None or not pipe.matches(in_sizes, out_sizes, chunk, _HOST_SLOTS):
+            self._pipe = None  # release the old staging first
+            pipe = self._pipe = _HostPipe(device, in_sizes, out_sizes, chunk, _HOST_SLOTS)
+        return pipe
+
+    def _inv_host_batch(self, geo, ctx, device, sl, a, x, obc_blocks) -> bool:
+        """One energy batch of host-resident stacks through ``_HostPipe``."""
+        a_, x_ = a.stack[sl], x.stack[sl]
+        E = int(a_.stack_shape[0])
+        host_in, host_out = list(a_.packed()), list(x_.packed())
+        pipe = self._pipe_for(device, tuple(int(t.shape[1]) for t in host_in),
+                              tuple(int(t.shape[1]) for t in host_out), E)
+        if pipe is None:
+            return False
+        has_obc = any(t is not None for t in obc_blocks.retarded)
+
+        def solve(c0, c1, din, dout):
+            obr = None
+            if has_obc:
+                obr = _obc_batch(obc_blocks.retarded, slice(sl.start + c0, sl.start + c1), geo, c1 - c0,
+                                 device)
+            _solve.selected_inv(geo, tuple(din), x=tuple(dout), obc_retarded=obr, ctx=ctx)
+
+        pipe.run(host_in, host_out, E, solve, ctx.stream)
+        ctx.status()
+        return True
+
     def _solve_host_batch(self, geo, ctx, device, sl, a, sigma_lesser, sigma_greater, xl, xg, xr,
                           obc_blocks, current) -> bool:
         """One energy batch of host-resident stacks through ``_HostPipe``;
@@ -361,15 +398,10 @@ class RGF(GFSolver):
         host_in = list(a_.packed()) + list(sl_.packed()[:2]) + list(sg_.packed()[:2])
         outs = [xl.stack[sl], xg.stack[sl]] + ([xr.stack[sl]] if xr is not None else [])
         host_out = [t for o in outs for t in o.packed()]
-        in_sizes = tuple(int(t.shape[1]) for t in host_in)
-        out_sizes = tuple(int(t.shape[1]) for t in host_out)
-        chunk = max(1, min((E + 1) // 2, _HOST_CHUNK_BYTES // (16 * sum(in_sizes))))
-        if E < 2:
+        pipe = self._pipe_for(device, tuple(int(t.shape[1]) for t in host_in),
+                              tuple(int(t.shape[1]) for t in host_out), E)
+        if pipe is None:
             return False
-        pipe = getattr(self, "_pipe", None)
-        if pipe is None or not pipe.matches(in_sizes, out_sizes, chunk, _HOST_SLOTS):
-            self._pipe = None  # release the old staging first
-            pipe = self._pipe = _HostPipe(device, in_sizes, out_sizes, chunk, _HOST_SLOTS)
         has_obc = any(t is not None for t in obc_blocks.retarded + obc_blocks.lesser + obc_blocks.greater)
         torch = _torch()
         cur = (torch.empty((E, geo.n_blocks + 1), dtype=torch.complex128, device=device)

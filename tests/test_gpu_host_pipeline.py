@@ -72,3 +72,20 @@ def test_host_pipeline_equals_device_solve(device, monkeypatch, E, max_batch, ch
             assert np.array_equal(ph.numpy(), pd.cpu().numpy())
     assert np.array_equal(np.asarray(cur_h), cur_d.cpu().numpy())
     assert np.abs(cur_d.cpu().numpy()).max() > 0
+
+
+@pytest.mark.parametrize("E,max_batch,chunk_bytes", [(6, 100, None), (7, 100, 1)])
+def test_host_pipeline_selected_inv_equals_device(device, monkeypatch, E, max_batch, chunk_bytes):
+    if chunk_bytes is not None:
+        monkeypatch.setattr(rgf_mod, "_HOST_CHUNK_BYTES", chunk_bytes)
+    bs, host, obc = _case(E, 71 + E)
+    A = host[0]
+    xh = BlockTridiagonal(bs, (E,))
+    solver = RGF(max_batch_size=max_batch)
+    solver.selected_inv(A, obc_blocks=obc, out=xh)
+    assert solver._pipe is not None, "the host-resident batch did not take the chunked path"
+    xd = BlockTridiagonal(bs, (E,), device=device)
+    RGF(max_batch_size=max_batch).selected_inv(_to(A, device), obc_blocks=obc, out=xd)
+    for ph, pd in zip(xh.packed(), xd.packed()):
+        assert np.array_equal(ph.numpy(), pd.cpu().numpy())
+    assert np.abs(xh.packed()[0].numpy()).max() > 0
