@@ -615,6 +615,7 @@ def obc_leg(args, dev) -> dict:
     torch.cuda.synchronize()
     tm = time.perf_counter() - t0
     out["memoized"] = {"ms": round(tm * 1e3, 3), "speedup_vs_fresh": round(t / tm, 2)}
+    out["spectral"] = spectral_run(args, dev)
     # size-independent self-check: the boundary self-energy of a converged
     # surface function has Gamma = i(S - S^H) Hermitian, and lesser + greater
     # = i (2f - 1) Gamma is anti-Hermitian
@@ -624,6 +625,48 @@ def obc_leg(args, dev) -> dict:
     out["lesser_antihermitian_rel"] = float(f"{herm:.3e}")
     out["sigma_norm"] = float(f"{float(torch.linalg.norm(sig[0])):.6e}")
     return out
+
+
+def spectral_run(args, dev) -> dict:
+    """The reference's default OBC (Spectral with Beyn's contour integral) on
+    a lead at the bench's block size: a 1-D chain (on-site 0, hopping 1) cut
+    into cells of b sites, energies inside its band, so each energy has two
+    finite non-zero modes, both on the unit circle (up to eta), inside the
+    annulus 0.5 < |w| < 2; the surface Green's function is checked against
+    the Sancho-Rubio decimation of the same lead."""
+    import torch
+
+    from quatrex_amd import workloads as W
+    from quatrex_amd.boundary_conditions import SanchoRubio, Spectral
+    from quatrex_amd.nevp import Beyn
+
+    E, b = args.obc_energies, args.block_size
+    z = torch.as_tensor(1.5 * W.draw_energies(E) + 1j * args.eta, device=dev)
+    eye = torch.eye(b, dtype=torch.complex128, device=dev)
+    h00 = torch.diag(torch.ones(b - 1, dtype=torch.complex128, device=dev), 1)
+    h00 = h00 + h00.T
+    h01 = torch.zeros((b, b), dtype=torch.complex128, device=dev)
+    h01[b - 1, 0] = 1.0
+    a_ii = (z[:, None, None] * eye - h00).contiguous()
+    a_ij = (-h01).expand(E, b, b).contiguous()
+    a_ji = (-h01.T).expand(E, b, b).contiguous()
+    r_o, r_i = 2.0, 0.5
+    sp = Spectral(Beyn(r_o=r_o, r_i=r_i, m_0=4, num_quad_points=32), max_decay=1.5 * np.log(r_o))
+    sp(a_ii, a_ij, a_ji, contact="left")  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = sp(a_ii, a_ij, a_ji, contact="left")
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ref = SanchoRubio(200, 1e-12)(a_ii, a_ij, a_ji, contact="left")
+    rel = float((torch.linalg.norm((x - ref).reshape(E, -1), dim=1) /
+                 torch.linalg.norm(ref.reshape(E, -1), dim=1)).max())
+    return {"workload": f"left contact, 1-D chain in cells of {b} sites, {E} energies in (-1.5, 1.5), "
+                        f"eta={args.eta}; Spectral(Beyn(r_o={r_o}, r_i={r_i}, m_0=4, "
+                        "num_quad_points=32), num_ref_iterations=2)",
+            "ms": round(t * 1e3, 3), "energies_per_s": round(E / t, 1),
+            "reflected_modes_min_max": [int(sp.n_reflected.min()), int(sp.n_reflected.max())],
+            "max_rel_diff_vs_sancho_rubio": float(f"{rel:.3e}")}
 
 
 def section_leg(args, dev) -> dict:
