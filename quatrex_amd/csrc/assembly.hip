@@ -7,9 +7,10 @@
 // _add_overlap (:541-565), _apply_potential (:567-609) and
 // _subtract_hamiltonian_and_self_energy (:611-665), in that operation order.
 //
-// One thread per element of one block: grid (elements of the largest block,
-// block, energy); the block table holds the packed offset, the shape and the
-// global row / column offsets of each of the 3N - 2 tridiagonal blocks.
+// One thread per element of one block for kEPT energies: grid (elements of
+// the largest block, block, energy group); the block table holds the packed
+// offset, the shape and the global row / column offsets of each of the 3N - 2
+// tridiagonal blocks.
 #include <algorithm>
 #include <string>
 #include <vector>
@@ -46,37 +47,47 @@ struct AsmArgs {
 
 __device__ inline double2 ld(const double2* p, int64_t i) { return p ? p[i] : cmk(0.0, 0.0); }
 
-__global__ void __launch_bounds__(256) assemble_system_kernel(AsmArgs P) {
+// Energies per thread: the element's index math and its shared inputs
+// (H when shared, overlap, potential) are loaded once for kEPT energies.
+constexpr int kEPT = 8;
+
+__global__ void __launch_bounds__(256) assemble_system_kernel(AsmArgs P, int n_energies) {
   const BlkInfo B = P.blk[blockIdx.y];
-  const int e = blockIdx.z;
-  const int64_t n = (int64_t)B.rows * B.cols;
+  const int e0 = blockIdx.z * kEPT;
+  const int ne = min(kEPT, n_energies - e0);
+  const int n = B.rows * B.cols;  // at most 1024 x 1024
   const int g = B.region;
-  const double2 z = P.z[e];
-  const int64_t se = (int64_t)e * P.rsz[g];
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(k / B.cols), c = (int)(k % B.cols);
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int r = k / B.cols, c = k - r * B.cols;
     const int64_t i = B.off + k;
     const int ra = B.row0 + r, cb = B.col0 + c;
-    // z S - P
-    double2 m;
-    if (P.s[g]) {
-      const double2 s = P.s[g][i];
-      m = cmul(z, s);
-      if (P.v) {
-        const double2 p1 = cmk(s.x * P.v[ra], s.y * P.v[ra]);
-        const double2 p2 = cmk(s.x * P.v[cb], s.y * P.v[cb]);
-        m = csub(m, cmk(0.5 * (p1.x + p2.x), 0.5 * (p1.y + p2.y)));
+    // the energy-independent parts of z S - P
+    const bool ovl = P.s[g] != nullptr;
+    const double2 s = ovl ? P.s[g][i] : cmk(0.0, 0.0);
+    const double pv = (!ovl && P.v && ra == cb) ? P.v[ra] : 0.0;  // diag(v) without an overlap
+    const double2 hs = P.h_stride[g] ? cmk(0.0, 0.0) : P.h[g][i];
+    for (int u = 0; u < ne; ++u) {
+      const int e = e0 + u;
+      const int64_t se = (int64_t)e * P.rsz[g];
+      const double2 z = P.z[e];
+      double2 m;
+      if (ovl) {
+        m = cmul(z, s);
+        if (P.v) {
+          const double2 p1 = cmk(s.x * P.v[ra], s.y * P.v[ra]);
+          const double2 p2 = cmk(s.x * P.v[cb], s.y * P.v[cb]);
+          m = csub(m, cmk(0.5 * (p1.x + p2.x), 0.5 * (p1.y + p2.y)));
+        }
+      } else {
+        m = (ra == cb) ? z : cmk(0.0, 0.0);
+        if (ra == cb) m = cmk(m.x - pv, m.y);
       }
-    } else {
-      m = (ra == cb) ? z : cmk(0.0, 0.0);
-      if (P.v && ra == cb) m = cmk(m.x - P.v[ra], m.y);
+      // - (Sigma^R_herm + (Sigma^> - Sigma^<) / 2 + H)
+      const double2 dl = csub(ld(P.sg[g], se + i), ld(P.sl[g], se + i));
+      double2 t = cadd(ld(P.sr[g], se + i), cmk(0.5 * dl.x, 0.5 * dl.y));
+      t = cadd(t, P.h_stride[g] ? P.h[g][(int64_t)e * P.h_stride[g] + i] : hs);
+      P.a[g][se + i] = csub(m, t);
     }
-    // - (Sigma^R_herm + (Sigma^> - Sigma^<) / 2 + H)
-    const double2 dl = csub(ld(P.sg[g], se + i), ld(P.sl[g], se + i));
-    double2 t = cadd(ld(P.sr[g], se + i), cmk(0.5 * dl.x, 0.5 * dl.y));
-    t = cadd(t, P.h[g][(int64_t)e * P.h_stride[g] + i]);
-    P.a[g][se + i] = csub(m, t);
   }
 }
 
@@ -146,8 +157,8 @@ extern "C" int rgf_assemble_system_z(rgf_ctx* ctx, const rgf_geometry* geom, int
   P.a[2] = static_cast<double2*>(a->lower);
   const int64_t bmax2 = g.bmax * g.bmax;
   const dim3 grid((unsigned)std::min<int64_t>(64, (bmax2 + 255) / 256), (unsigned)blk.size(),
-                  (unsigned)n_energies);
-  hipLaunchKernelGGL(assemble_system_kernel, grid, dim3(256), 0, ctx->stream, P);
+                  (unsigned)((n_energies + kEPT - 1) / kEPT));
+  hipLaunchKernelGGL(assemble_system_kernel, grid, dim3(256), 0, ctx->stream, P, (int)n_energies);
   return ctx->hip(hipGetLastError(), "assemble system");
   GUARD_END
 }
