@@ -91,7 +91,7 @@ def _rows(path: Path):
 
 def _short(name: str) -> str:
     """Kernel name without the return type, namespaces and argument list."""
-    for pre in ("void ", "qx::(anonymous namespace)::", "qx::"):
+    for pre in ("void ", "qx::(anonymous namespace)::", "qx::", "(anonymous namespace)::"):
         if name.startswith(pre):
             name = name[len(pre):]
     name = name.replace("qx::(anonymous namespace)::", "")
