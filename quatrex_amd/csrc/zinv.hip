@@ -495,41 +495,61 @@ zinv_panel2_kernel(double2* S, int64_t sS, int n, int k, int nbe, char* aux, int
   }
 }
 
-// Rows staged per workgroup of the finish kernel: rows * n <= kFinishElems.
+// Rows staged per workgroup of the finish kernel: rows * n <= kFinishElems,
+// rows <= kFinishRows; kFinishThreads threads per workgroup.
 constexpr int kFinishElems = 2048;
+constexpr int kFinishRows = 256;
+constexpr int kFinishThreads = 256;
 
 template <int NB>
 __global__ void __launch_bounds__(256)
 zinv_finish_kernel(const double2* S, int64_t sS, int n, const char* aux, int64_t sAux,
                    double2* Out, int64_t sOut, int ldo, int rows_per_wg) {
   __shared__ int colsrc[1024];
+  __shared__ int rowsrc[kFinishRows];
   __shared__ double2 rows_lds[kFinishElems];
   const int e = blockIdx.y;
   const char* ax = aux + (int64_t)e * sAux;
   const int* piv = reinterpret_cast<const int*>(ax + PanelAux<NB>::piv(n));
   const int* step_of = reinterpret_cast<const int*>(ax + PanelAux<NB>::step(n));
+  const int r0 = blockIdx.x * rows_per_wg;
   // (the clamps only matter for NaN input, where the pivot order is meaningless)
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const int s = step_of[t];
     colsrc[t] = (s >= 0 && s < n) ? s : 0;
   }
+  for (int t = threadIdx.x; t < rows_per_wg; t += blockDim.x) {
+    const int pr = (r0 + t < n) ? piv[r0 + t] : 0;
+    rowsrc[t] = (pr >= 0 && pr < n) ? pr : 0;
+  }
+  __syncthreads();
   const double2* A = S + (int64_t)e * sS;
   double2* O = Out + (int64_t)e * sOut;
-  const int r0 = blockIdx.x * rows_per_wg;
   const int total = rows_per_wg * n;
-  // source rows pi(r) read whole and coalesced ...
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    const int r = r0 + idx / n, t = idx % n;
-    if (r < n) {
-      const int pr = piv[r];
-      rows_lds[idx] = A[(int64_t)((pr >= 0 && pr < n) ? pr : 0) * n + t];
-    }
+  // source rows pi(r) read whole and coalesced, all of a thread's loads in
+  // flight before their LDS stores ...
+  constexpr int kU = kFinishElems / kFinishThreads;
+  double2 v[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int idx = threadIdx.x + u * kFinishThreads;
+    const int rr = idx / n, t = idx - rr * n;
+    // (indexes clamped so that even a speculated LDS read stays in its array)
+    v[u] = (idx < total && r0 + rr < n) ? A[(int64_t)rowsrc[min(rr, kFinishRows - 1)] * n + t]
+                                        : cmk(0.0, 0.0);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int idx = threadIdx.x + u * kFinishThreads;
+    if (idx < total) rows_lds[idx] = v[u];
   }
   __syncthreads();
   // ... and permuted by column inside LDS: Out[r][j] = S[pi(r)][pi^{-1}(j)]
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    const int rr = idx / n, j = idx % n, r = r0 + rr;
-    if (r < n) O[(int64_t)r * ldo + j] = rows_lds[rr * n + colsrc[j]];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int idx = threadIdx.x + u * kFinishThreads;
+    const int rr = idx / n, j = idx - rr * n, r = r0 + rr;
+    if (idx < total && r < n) O[(int64_t)r * ldo + j] = rows_lds[min(rr * n + colsrc[j], kFinishElems - 1)];
   }
 }
 
@@ -625,7 +645,7 @@ hipError_t zinv_run(double2* S, int64_t sS, int n, int nE, double2* Out, int64_t
     k = k3;
   }
   if (err != hipSuccess) return err;
-  const int rows = kFinishElems / n;  // n <= 1024: at least 2
+  const int rows = std::min(kFinishElems / n, kFinishRows);  // n <= 1024: at least 2
   hipLaunchKernelGGL((zinv_finish_kernel<NB>), dim3((n + rows - 1) / rows, nE), dim3(256), 0,
                      stream, S, sS, n, ax, sAux, Out, sOut, ldo, rows);
   return hipGetLastError();
