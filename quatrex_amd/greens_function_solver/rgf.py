@@ -227,6 +227,11 @@ class _HostPipe:
         self.ev_out = [torch.cuda.Event() for _ in S]
         self.pool = ThreadPoolExecutor(max_workers=1)
 
+    def __del__(self):
+        pool = getattr(self, "pool", None)
+        if pool is not None:
+            pool.shutdown(wait=False)
+
     def matches(self, in_sizes, out_sizes, chunk, slots) -> bool:
         return (self.in_sizes, self.out_sizes, self.chunk, self.slots) == (in_sizes, out_sizes, chunk, slots)
 
