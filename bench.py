@@ -876,10 +876,38 @@ def _lib_opt(name: str) -> int:
     return getattr(_lib, name)
 
 
+def _host_tridiagonal_dsdb(g, packed, rows_cols=None):
+    """A CPU block-sparse stack (DSDBCOO, torch data) of the full tridiagonal
+    pattern holding packed (diag, upper, lower) numpy stacks."""
+    import torch
+
+    from quatrex_amd import dsdb as D
+
+    d, u, l = packed
+    E = d.shape[0]
+    bs = list(g.block_sizes)
+    if rows_cols is None:
+        rows, cols, _ = _tridiagonal_coo(g, (d[:1], u[:1], l[:1]))
+    else:
+        rows, cols = rows_cols
+    data = torch.empty((E, rows.size), dtype=torch.complex128)
+    k = 0
+    for i in range(g.n_blocks):
+        for j, part in ((i - 1, "l"), (i, "d"), (i + 1, "u")):
+            if j < 0 or j >= g.n_blocks:
+                continue
+            blk = g.diag(d, i) if part == "d" else (g.upper(u, i) if part == "u" else g.lower(l, j))
+            n = bs[i] * bs[j]
+            data[:, k:k + n] = torch.from_numpy(np.ascontiguousarray(blk).reshape(E, n))
+            k += n
+    return D.DSDBCOO(data, rows, cols, g.block_sizes, (E,)), (rows, cols)
+
+
 def host_io_leg(args, dev) -> dict:
-    """The drop-in RGF on host-resident (CPU torch) block-tridiagonal stacks:
-    blocks gathered on the host, staged through pinned memory to the device,
-    solved, and written back block by block - the PCIe-inclusive rate of a
+    """The drop-in RGF on host-resident (CPU torch) stacks, block-tridiagonal
+    and block-sparse (full tridiagonal pattern): energy chunks staged through
+    pinned memory, moved over PCIe and solved with the copies of both
+    directions and the solves overlapping - the PCIe-inclusive rate of a
     caller that hands over host buffers (the device-resident rate is `value`)."""
     import torch
 
@@ -908,11 +936,35 @@ def host_io_leg(args, dev) -> dict:
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     moved = 16 * E * (3 * (g.diag_size + 2 * g.offdiag_size) + 3 * (g.diag_size + 2 * g.offdiag_size))
-    return {"workload": f"{g.n_blocks} blocks x {args.block_size}, {E} energies, complex128, CPU-resident "
-                        "A, Sigma^<, Sigma^> and outputs (G^R, G^<, G^> all tridiagonal blocks)",
-            "energies_per_s": round(E / el, 2), "ms": round(el * 1e3, 1),
-            "host_bytes_in_and_out": moved,
-            "effective_gbs": round(moved / el / 1e9, 2)}
+    res = {"workload": f"{g.n_blocks} blocks x {args.block_size}, {E} energies, complex128, CPU-resident "
+                       "A, Sigma^<, Sigma^> and outputs (G^R, G^<, G^> all tridiagonal blocks)",
+           "energies_per_s": round(E / el, 2), "ms": round(el * 1e3, 1),
+           "host_bytes_in_and_out": moved,
+           "effective_gbs": round(moved / el / 1e9, 2)}
+    nl = g.block_sizes[-1] ** 2
+    ref = outs[1].packed()[0][-1, -nl:].clone()  # G^< of the last energy's last diagonal block
+    del A, SL, SG, outs, solver
+    # the same solve on block-sparse stacks (DSDBCOO, CPU torch data): the
+    # stored entries move, (de)densified on the device per chunk
+    a, rc = _host_tridiagonal_dsdb(g, (ad, au, al))
+    del ad, au, al
+    z = np.zeros_like(slu)
+    sl, _ = _host_tridiagonal_dsdb(g, (sld, slu, z), rc)
+    sg, _ = _host_tridiagonal_dsdb(g, (sgd, sgu, z), rc)
+    del sld, slu, sgd, sgu, z
+    xs = [type(a).zeros_like(a) for _ in range(3)]
+    solver = RGF(max_batch_size=E)
+    solver.selected_solve(a, sl, sg, out=xs, return_retarded=True)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solver.selected_solve(a, sl, sg, out=xs, return_retarded=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    # the last stored entries are the last diagonal block, row by row
+    res["block_sparse"] = {"energies_per_s": round(E / el, 2), "ms": round(el * 1e3, 1),
+                           "stored_entries_per_energy": int(a.nnz),
+                           "equal_to_block_tridiagonal_run": bool(torch.equal(xs[1].data[-1, -nl:], ref))}
+    return res
 
 
 def serving_leg(args, dev) -> dict:

@@ -324,6 +324,24 @@ def device_map(m, geo: Geometry, device) -> SB.DeviceMap:
     return cache[key]
 
 
+def writes_every_entry(m, geo: Geometry, parts: str, nnz: int, device) -> bool:
+    """Whether writing the packed ``parts`` into ``m`` (rgf_sparsify_z)
+    overwrites every one of its ``nnz`` stored entries, so that an output's
+    previous values need not be read (cached per structure and parts)."""
+    torch = _torch()
+    dmap = device_map(m, geo, device)
+    cache = dmap.__dict__.setdefault("_writes_all", {})
+    key = (parts, nnz)
+    if key not in cache:
+        seen = torch.zeros(nnz, dtype=torch.bool, device=device)
+        for k in parts:
+            src, _, op = dmap.tensors[k]
+            if src.numel():
+                seen[src[(op & SB.OP_MIRROR) == 0]] = True
+        cache[key] = bool(seen.all())
+    return cache[key]
+
+
 def _batch_data(m, sl, device):
     """The batch's data as a contiguous ``(E, nnz)`` complex128 device tensor."""
     torch = _torch()
@@ -339,11 +357,16 @@ def _batch_data(m, sl, device):
     return t.reshape(-1, nnz).contiguous(), lead
 
 
-def densify_batch(m, sl, geo: Geometry, parts: str, device, ctx):
+def densify_batch(m, sl, geo: Geometry, parts: str, device, ctx, data=None):
     """Packed ``(diag, upper, lower)`` of a batch of a block-sparse stack: one
-    densify launch per part for all energies (rgf_densify_z)."""
+    densify launch per part for all energies (rgf_densify_z).  ``data``: the
+    batch's stored entries already on the device (``(E, nnz)``), else they are
+    read from ``m.data[sl]``."""
     torch = _torch()
-    data, lead = _batch_data(m, sl, device)
+    if data is None:
+        data, lead = _batch_data(m, sl, device)
+    else:
+        lead = (int(data.shape[0]),)
     E = int(data.shape[0])
     sizes = {"d": geo.diag_size, "u": geo.offdiag_size, "l": geo.offdiag_size}
     out = {k: torch.empty((E, sizes[k]), dtype=torch.complex128, device=device) for k in parts}
@@ -358,13 +381,18 @@ def densify_batch(m, sl, geo: Geometry, parts: str, device, ctx):
     return tuple(out.get(k) for k in "dul"), lead
 
 
-def sparsify_batch(m, sl, geo: Geometry, bufs, parts: str, device, ctx) -> None:
+def sparsify_batch(m, sl, geo: Geometry, bufs, parts: str, device, ctx, data=None) -> None:
     """Write packed results into the stored entries of a block-sparse stack
-    batch (rgf_sparsify_z); entries outside the written blocks keep their values."""
+    batch (rgf_sparsify_z); entries outside the written blocks keep their
+    values.  ``data``: a device ``(E, nnz)`` tensor holding the batch's
+    stored entries, written in place instead of ``m.data[sl]``."""
     torch = _torch()
-    view = m.data[sl]
-    direct = (isinstance(view, torch.Tensor) and view.device == torch.device(device)
-              and view.dtype == torch.complex128 and view.is_contiguous())
+    if data is not None:
+        view, direct = data, True
+    else:
+        view = m.data[sl]
+        direct = (isinstance(view, torch.Tensor) and view.device == torch.device(device)
+                  and view.dtype == torch.complex128 and view.is_contiguous())
     if direct:
         data = view.reshape(-1, view.shape[-1])
     else:

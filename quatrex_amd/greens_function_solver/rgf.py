@@ -33,6 +33,7 @@ import numpy as np
 from quatrex_amd import solve as _solve
 from quatrex_amd.dsdb import densify_batch as _gather_dsdb
 from quatrex_amd.dsdb import sparsify_batch as _scatter_dsdb
+from quatrex_amd.dsdb import writes_every_entry as _dsdb_writes_all
 from quatrex_amd.dsdb import structure_of as _structure_of
 from quatrex_amd.datastructures import BlockTridiagonal
 from quatrex_amd.greens_function_solver.solver import GFSolver, OBCBlocks
@@ -181,6 +182,15 @@ def _zeros_current(like_container, shape):
 # that each chunk's solve keeps the GPU busy, small enough that the staging
 # of all slots stays modest (4 energies of 32 x 256: ~0.9 GB in, ~1.2 GB out).
 _HOST_CHUNK_BYTES = 1 << 30
+# Fewest energies per chunk when block-sparse operands take part.  Their
+# chunks also carry the (de)densification launches and packed buffers, and
+# a solve of a few energies costs nearly as much as one of eight (the block
+# inversions' panel eliminations take one workgroup per matrix), so small
+# chunks leave such a pipeline bound by the solves instead of by PCIe: at
+# 32 blocks x 256, 64 energies, 4 energies per chunk ran 62 energies/s, 8
+# ran 80.7.  Block-tridiagonal stacks keep the byte-sized chunks (4 energies
+# there: 88 energies/s, against 83.6 with 8).
+_HOST_MIN_CHUNK = 8
 # Staging slots: chunks in flight between the host copies, the transfers and
 # the solves.
 _HOST_SLOTS = 3
@@ -193,6 +203,24 @@ def _host_packed(m) -> bool:
     return (isinstance(m, BlockTridiagonal) and m.device.type == "cpu" and not m.symmetry
             and m.dtype == torch.complex128 and len(m.stack_shape) == 1
             and all(t.is_contiguous() for t in m.packed(flatten=False)))
+
+
+def _host_dsdb(m) -> bool:
+    """A block-sparse stack whose stored entries are a host array ``(energies,
+    nnz)`` of complex128 in C order, numpy or torch (a contiguous substack
+    view of one included)."""
+    if not _is_dsdb(m):
+        return False
+    d, torch = m.data, _torch()
+    if isinstance(d, torch.Tensor):
+        return (d.device.type == "cpu" and d.dtype == torch.complex128 and d.dim() == 2
+                and d.is_contiguous())
+    return (isinstance(d, np.ndarray) and d.dtype == np.complex128 and d.ndim == 2
+            and d.flags.c_contiguous)
+
+
+def _host_operand(m) -> bool:
+    return _host_packed(m) or _host_dsdb(m)
 
 
 class _HostPipe:
@@ -332,7 +360,7 @@ class RGF(GFSolver):
             obc_blocks = OBCBlocks(num_blocks=a.num_blocks)
         _, offs = get_batches(a.shape[0], self.max_batch_size)
         x = out if out is not None else a.__class__.zeros_like(a)
-        host = _host_packed(a) and _host_packed(x)
+        host = _host_operand(a) and _host_operand(x)
         for bidx in range(len(offs) - 1):
             sl = slice(int(offs[bidx]), int(offs[bidx + 1]))
             if host and self._inv_host_batch(geo, ctx, device, sl, a, x, obc_blocks):
@@ -360,10 +388,10 @@ class RGF(GFSolver):
         return None
 
     # ------------------------------------------------------------------
-    def _pipe_for(self, device, in_sizes, out_sizes, E):
+    def _pipe_for(self, device, in_sizes, out_sizes, E, min_chunk=1):
         """The staging of this solver for a batch of E host-resident energies
         (None when the batch is too small to split)."""
-        chunk = max(1, min((E + 1) // 2, _HOST_CHUNK_BYTES // (16 * sum(in_sizes))))
+        chunk = max(1, min((E + 1) // 2, max(min_chunk, _HOST_CHUNK_BYTES // (16 * sum(in_sizes)))))
         if E < 2:
             return None
         pipe = getattr(self, "_pipe", None)
@@ -372,47 +400,101 @@ class RGF(GFSolver):
             pipe = self._pipe = _HostPipe(device, in_sizes, out_sizes, chunk, _HOST_SLOTS)
         return pipe
 
-    def _inv_host_batch(self, geo, ctx, device, sl, a, x, obc_blocks) -> bool:
-        """One energy batch of host-resident stacks through ``_HostPipe``."""
-        a_, x_ = a.stack[sl], x.stack[sl]
-        E = int(a_.stack_shape[0])
-        host_in, host_out = list(a_.packed()), list(x_.packed())
+    def _host_batch(self, geo, ctx, device, sl, ins, outs, solve) -> bool:
+        """One energy batch of host-resident operands through ``_HostPipe``;
+        False (nothing done) when the batch is too small to split.
+
+        ins: ``(stack, parts)`` pairs; outs: ``(stack, parts, written)``
+        triples (``written``: the parts stored back, a subset of ``parts``).
+        A host BlockTridiagonal moves its packed parts; a host block-sparse
+        stack moves its stored entries, densified into packed
+        buffers (inputs) or sparsified from them (outputs) on the device per
+        chunk.  A block-sparse output whose written parts do not hold every
+        stored entry is staged in as well, so that the entries outside them
+        keep their values.  ``solve(c0, c1, packed_in, packed_out)`` enqueues
+        the solve of energies [c0, c1) on the context stream, given
+        ``(d, u, l)`` device tuples (None for parts not listed) of ins and
+        outs.
+        """
+        torch = _torch()
+        sizes = {"d": geo.diag_size, "u": geo.offdiag_size, "l": geo.offdiag_size}
+        host_in, host_out, spec_in, spec_out, keep = [], [], [], [], []
+
+        def host_parts(m, parts):
+            if _is_dsdb(m):
+                d = m.data[sl]
+                return [d if isinstance(d, torch.Tensor) else torch.from_numpy(d)], True
+            p = m.stack[sl].packed()
+            return [p["dul".index(k)] for k in parts], False
+
+        for m, parts in ins:
+            ts, sparse = host_parts(m, parts)
+            spec_in.append((m, parts, parts, sparse, len(host_in)))
+            host_in += ts
+        for m, parts, written in outs:
+            ts, sparse = host_parts(m, parts)
+            spec_out.append((m, parts, written, sparse, len(host_out)))
+            if sparse and not _dsdb_writes_all(m, geo, written, int(ts[0].shape[1]), device):
+                keep.append((len(host_in), len(host_out)))
+                host_in.append(ts[0])
+            host_out += ts
+        E = int(host_in[0].shape[0])
+        any_sparse = any(sp[3] for sp in spec_in + spec_out)
         pipe = self._pipe_for(device, tuple(int(t.shape[1]) for t in host_in),
-                              tuple(int(t.shape[1]) for t in host_out), E)
+                              tuple(int(t.shape[1]) for t in host_out), E,
+                              _HOST_MIN_CHUNK if any_sparse else 1)
         if pipe is None:
             return False
+
+        def packed(spec, bufs, n, inputs):
+            res, post = [], []
+            for m, parts, written, sparse, k in spec:
+                if not sparse:
+                    res.append(tuple(bufs[k + parts.index(c)] if c in parts else None for c in "dul"))
+                elif inputs:
+                    res.append(_gather_dsdb(m, None, geo, parts, device, ctx, data=bufs[k])[0])
+                else:
+                    P = tuple(torch.empty((n, sizes[c]), dtype=torch.complex128, device=device)
+                              if c in parts else None for c in "dul")
+                    res.append(P)
+                    post.append((m, P, written, bufs[k]))
+            return res, post
+
+        def chunk(c0, c1, din, dout):
+            for i, j in keep:  # the stored entries outside the written blocks
+                dout[j].copy_(din[i])
+            pin, _ = packed(spec_in, din, c1 - c0, True)
+            pout, post = packed(spec_out, dout, c1 - c0, False)
+            solve(c0, c1, pin, pout)
+            for m, P, written, d in post:
+                _scatter_dsdb(m, None, geo, P, written, device, ctx, data=d)
+
+        pipe.run(host_in, host_out, E, chunk, ctx.stream)
+        ctx.status()
+        return True
+
+    def _inv_host_batch(self, geo, ctx, device, sl, a, x, obc_blocks) -> bool:
         has_obc = any(t is not None for t in obc_blocks.retarded)
 
-        def solve(c0, c1, din, dout):
+        def solve(c0, c1, pin, pout):
             obr = None
             if has_obc:
                 obr = _obc_batch(obc_blocks.retarded, slice(sl.start + c0, sl.start + c1), geo, c1 - c0,
                                  device)
-            _solve.selected_inv(geo, tuple(din), x=tuple(dout), obc_retarded=obr, ctx=ctx)
+            _solve.selected_inv(geo, pin[0], x=pout[0], obc_retarded=obr, ctx=ctx)
 
-        pipe.run(host_in, host_out, E, solve, ctx.stream)
-        ctx.status()
-        return True
+        return self._host_batch(geo, ctx, device, sl, [(a, "dul")],
+                                [(x, "dul", "du" if x.symmetry else "dul")], solve)
 
     def _solve_host_batch(self, geo, ctx, device, sl, a, sigma_lesser, sigma_greater, xl, xg, xr,
                           obc_blocks, current) -> bool:
-        """One energy batch of host-resident stacks through ``_HostPipe``;
-        False (nothing done) when the batch is too small to split."""
-        a_, sl_, sg_ = a.stack[sl], sigma_lesser.stack[sl], sigma_greater.stack[sl]
-        E = int(a_.stack_shape[0])
-        host_in = list(a_.packed()) + list(sl_.packed()[:2]) + list(sg_.packed()[:2])
-        outs = [xl.stack[sl], xg.stack[sl]] + ([xr.stack[sl]] if xr is not None else [])
-        host_out = [t for o in outs for t in o.packed()]
-        pipe = self._pipe_for(device, tuple(int(t.shape[1]) for t in host_in),
-                              tuple(int(t.shape[1]) for t in host_out), E)
-        if pipe is None:
-            return False
         has_obc = any(t is not None for t in obc_blocks.retarded + obc_blocks.lesser + obc_blocks.greater)
         torch = _torch()
+        E = sl.stop - sl.start
         cur = (torch.empty((E, geo.n_blocks + 1), dtype=torch.complex128, device=device)
                if current is not None else None)
 
-        def solve(c0, c1, din, dout):
+        def solve(c0, c1, pin, pout):
             n = c1 - c0
             obc = None
             if has_obc:
@@ -420,13 +502,17 @@ class RGF(GFSolver):
                 obc = (_obc_batch(obc_blocks.retarded, csl, geo, n, device),
                        _obc_batch(obc_blocks.lesser, csl, geo, n, device),
                        _obc_batch(obc_blocks.greater, csl, geo, n, device))
-            _solve.selected_solve(geo, tuple(din[0:3]), (din[3], din[4]), (din[5], din[6]),
-                                  g_lesser=tuple(dout[0:3]), g_greater=tuple(dout[3:6]),
-                                  g_retarded=tuple(dout[6:9]) if xr is not None else None, obc=obc,
-                                  current=cur[c0:c1] if cur is not None else None, ctx=ctx)
+            _solve.selected_solve(geo, pin[0], pin[1][:2], pin[2][:2], g_lesser=pout[0],
+                                  g_greater=pout[1], g_retarded=pout[2] if xr is not None else None,
+                                  obc=obc, current=cur[c0:c1] if cur is not None else None, ctx=ctx)
 
-        pipe.run(host_in, host_out, E, solve, ctx.stream)
-        ctx.status()
+        part = lambda m, sym: (sym, sym) if m.symmetry else ("dul", "dul")  # noqa: E731
+        outs = [(xl, *part(xl, "du")), (xg, *part(xg, "du"))]
+        if xr is not None:
+            outs.append((xr, *part(xr, "d")))
+        if not self._host_batch(geo, ctx, device, sl, [(a, "dul"), (sigma_lesser, "du"),
+                                                       (sigma_greater, "du")], outs, solve):
+            return False
         if cur is not None:
             current[sl] = cur if not isinstance(current, np.ndarray) else cur.cpu().numpy()
         return True
@@ -458,8 +544,8 @@ class RGF(GFSolver):
             if return_retarded:
                 xr = a.__class__.zeros_like(a)
 
-        host = (_host_packed(a) and _host_packed(sigma_lesser) and _host_packed(sigma_greater)
-                and _host_packed(xl) and _host_packed(xg) and (not return_retarded or _host_packed(xr)))
+        host = (_host_operand(a) and _host_operand(sigma_lesser) and _host_operand(sigma_greater)
+                and _host_operand(xl) and _host_operand(xg) and (not return_retarded or _host_operand(xr)))
         for bidx in range(len(offs) - 1):
             sl = slice(int(offs[bidx]), int(offs[bidx + 1]))
             if host and self._solve_host_batch(geo, ctx, device, sl, a, sigma_lesser, sigma_greater,

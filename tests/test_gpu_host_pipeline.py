@@ -55,6 +55,7 @@ def test_host_pipeline_equals_device_solve(device, monkeypatch, E, max_batch, ch
     if chunk_bytes is not None:
         # one energy per chunk: every staging slot is reused several times
         monkeypatch.setattr(rgf_mod, "_HOST_CHUNK_BYTES", chunk_bytes)
+        monkeypatch.setattr(rgf_mod, "_HOST_MIN_CHUNK", 1)
     bs, host, obc = _case(E, 41 + E)
     outs_h = tuple(BlockTridiagonal(bs, (E,)) for _ in range(3))
     solver = RGF(max_batch_size=max_batch)
@@ -78,6 +79,7 @@ def test_host_pipeline_equals_device_solve(device, monkeypatch, E, max_batch, ch
 def test_host_pipeline_selected_inv_equals_device(device, monkeypatch, E, max_batch, chunk_bytes):
     if chunk_bytes is not None:
         monkeypatch.setattr(rgf_mod, "_HOST_CHUNK_BYTES", chunk_bytes)
+        monkeypatch.setattr(rgf_mod, "_HOST_MIN_CHUNK", 1)
     bs, host, obc = _case(E, 71 + E)
     A = host[0]
     xh = BlockTridiagonal(bs, (E,))
@@ -89,3 +91,119 @@ def test_host_pipeline_selected_inv_equals_device(device, monkeypatch, E, max_ba
     for ph, pd in zip(xh.packed(), xd.packed()):
         assert np.array_equal(ph.numpy(), pd.cpu().numpy())
     assert np.abs(xh.packed()[0].numpy()).max() > 0
+
+
+# --- host-resident block-sparse stacks -------------------------------------
+
+def _sparse_case(kind, E, seed, extra):
+    """System matrix, Sigma^<> (symmetric storage) and three outputs as host
+    block-sparse stacks of layout ``kind``: the pattern holds the tridiagonal
+    blocks and, with ``extra``, entries outside them (which the solve must
+    leave as they were in the outputs).  The outputs start from random
+    values."""
+    from scipy import sparse
+
+    from oracle.dsdb import OracleDSDB
+    from quatrex_amd.dsdb import DSDBCOO, DSDBCSR
+    from tests.helpers import tridiagonal_mask
+
+    bs = (8, 12, 8, 10, 6)
+    g = Geometry(bs)
+    model = W.make_device(g, seed)
+    A = g.to_dense(*W.system_matrix(model, W.draw_energies(E), 1e-2))
+    sld, slu, sgd, sgu = W.sigmas(model, E)
+    z = np.zeros_like(slu)
+    L, G = g.to_dense(sld, slu, z), g.to_dense(sgd, sgu, z)
+    mask = tridiagonal_mask(list(bs))
+    if extra:
+        mask[0, -3:] = mask[-2:, 1] = True  # entries in blocks (0, 4) and (4, 0)
+    rng = np.random.default_rng(seed)
+    skew = lambda x: -np.conj(x)  # noqa: E731
+
+    def stack(dense, symmetry, op):
+        pat = np.triu(mask) if symmetry else mask
+        if kind == "oracle":
+            r, c = np.nonzero(pat)
+            o = OracleDSDB(np.zeros((E, r.size), np.complex128), r, c, bs, symmetry, op)
+            o.data[...] = dense[:, r, c]
+            return o
+        layout = DSDBCOO if kind == "coo" else DSDBCSR
+        m = layout.from_sparray(sparse.coo_matrix(pat.astype(np.complex128)), bs, (E,), symmetry, op,
+                                device="cpu")
+        r, c, pos = m._coordinates()
+        vals = np.zeros((E, m.nnz), np.complex128)
+        vals[:, pos] = dense[:, r, c]
+        m.data.copy_(__import__("torch").from_numpy(vals))
+        return m
+
+    def noise():
+        return rng.standard_normal(A.shape) + 1j * rng.standard_normal(A.shape)
+
+    ins = (stack(A, False, np.conj), stack(L, True, skew), stack(G, True, skew))
+    outs = (stack(noise(), True, skew), stack(noise(), True, skew), stack(noise(), False, np.conj))
+    return bs, ins, outs
+
+
+def _data(m):
+    d = m.data
+    return d.numpy().copy() if hasattr(d, "numpy") else np.array(d)
+
+
+@pytest.mark.parametrize("kind", ["coo", "csr", "oracle"])
+@pytest.mark.parametrize("extra,chunk_bytes", [(False, None), (True, 1)])
+def test_host_pipeline_block_sparse_equals_unpipelined(device, monkeypatch, kind, extra, chunk_bytes):
+    """Host block-sparse stacks through the chunked staging (batches of 7)
+    against the same stacks solved one energy per batch (no chunking: the
+    batch's entries staged once), bit for bit, entries outside the
+    tridiagonal blocks of the outputs included."""
+    if chunk_bytes is not None:
+        monkeypatch.setattr(rgf_mod, "_HOST_CHUNK_BYTES", chunk_bytes)
+        monkeypatch.setattr(rgf_mod, "_HOST_MIN_CHUNK", 1)
+    E = 7
+    bs, ins, outs_p = _sparse_case(kind, E, 611, extra)
+    _, _, outs_r = _sparse_case(kind, E, 611, extra)
+    before = [_data(o) for o in outs_p]
+    rng = np.random.default_rng(3)
+
+    def contact(b, f):
+        br = -0.5j * np.eye(b) + 0.05 * (rng.standard_normal((E, b, b)) + 1j * rng.standard_normal((E, b, b)))
+        gam = 1j * (br - np.conj(np.swapaxes(br, -1, -2)))
+        return br, 1j * gam * f, 1j * gam * (f - 1)
+
+    Lc, Rc = contact(bs[0], 0.7), contact(bs[-1], 0.2)
+    obc = OBCBlocks(num_blocks=len(bs))
+    for k, lst in enumerate((obc.retarded, obc.lesser, obc.greater)):
+        lst[0], lst[-1] = Lc[k], Rc[k]
+    solver = RGF(max_batch_size=100)
+    cur_p = solver.selected_solve(*ins, obc_blocks=obc, out=outs_p, return_retarded=True,
+                                  return_current=True)
+    assert solver._pipe is not None, "the host-resident batch did not take the chunked path"
+    cur_r = RGF(max_batch_size=1).selected_solve(*ins, obc_blocks=obc, out=outs_r, return_retarded=True,
+                                                 return_current=True)
+    for op, orf, b in zip(outs_p, outs_r, before):
+        got = _data(op)
+        assert np.array_equal(got, _data(orf))
+        assert not np.array_equal(got, b)
+    assert np.array_equal(np.asarray(cur_p), np.asarray(cur_r))
+    if extra:
+        # the entries outside the tridiagonal blocks kept their values
+        for op, b in zip(outs_p, before):
+            rr, cc, pos = (op.rows, op.cols, np.arange(op.rows.size)) if kind == "oracle" \
+                else op._coordinates()
+            off = np.abs(np.repeat(np.arange(len(bs)), bs)[rr] - np.repeat(np.arange(len(bs)), bs)[cc]) > 1
+            assert off.any()
+            assert np.array_equal(_data(op)[:, pos[off]], b[:, pos[off]])
+
+
+@pytest.mark.parametrize("kind", ["coo", "oracle"])
+def test_host_pipeline_block_sparse_selected_inv(device, monkeypatch, kind):
+    monkeypatch.setattr(rgf_mod, "_HOST_CHUNK_BYTES", 1)
+    monkeypatch.setattr(rgf_mod, "_HOST_MIN_CHUNK", 1)
+    E = 5
+    _, ins, outs_p = _sparse_case(kind, E, 733, True)
+    _, _, outs_r = _sparse_case(kind, E, 733, True)
+    solver = RGF(max_batch_size=100)
+    solver.selected_inv(ins[0], out=outs_p[2])
+    assert solver._pipe is not None, "the host-resident batch did not take the chunked path"
+    RGF(max_batch_size=1).selected_inv(ins[0], out=outs_r[2])
+    assert np.array_equal(_data(outs_p[2]), _data(outs_r[2]))
