@@ -935,7 +935,10 @@ def host_io_leg(args, dev) -> dict:
     solver.selected_solve(A, SL, SG, out=outs, return_retarded=True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    moved = 16 * E * (3 * (g.diag_size + 2 * g.offdiag_size) + 3 * (g.diag_size + 2 * g.offdiag_size))
+    full = g.diag_size + 2 * g.offdiag_size  # all three parts
+    # in: A (3 parts), Sigma^<> (diag and upper: the lower parts are not
+    # read); out: G^R, G^<, G^> (3 parts each)
+    moved = 16 * E * (full + 2 * (g.diag_size + g.offdiag_size) + 3 * full)
     res = {"workload": f"{g.n_blocks} blocks x {args.block_size}, {E} energies, complex128, CPU-resident "
                        "A, Sigma^<, Sigma^> and outputs (G^R, G^<, G^> all tridiagonal blocks)",
            "energies_per_s": round(E / el, 2), "ms": round(el * 1e3, 1),
@@ -961,8 +964,11 @@ def host_io_leg(args, dev) -> dict:
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     # the last stored entries are the last diagonal block, row by row
+    moved_sp = 16 * E * 6 * int(a.nnz)  # every stored entry of the three inputs and three outputs
     res["block_sparse"] = {"energies_per_s": round(E / el, 2), "ms": round(el * 1e3, 1),
                            "stored_entries_per_energy": int(a.nnz),
+                           "host_bytes_in_and_out": moved_sp,
+                           "effective_gbs": round(moved_sp / el / 1e9, 2),
                            "equal_to_block_tridiagonal_run": bool(torch.equal(xs[1].data[-1, -nl:], ref))}
     return res
 
